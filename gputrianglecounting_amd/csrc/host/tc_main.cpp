@@ -1,0 +1,220 @@
+// tc_main — command-line driver of the gfx950 triangle counter.
+//
+// Counterpart of the reference binary bin/ccgpunew (src/main.cu:1-38,
+// src/gpurun.cu:2-56, 115-176, src/param.cu:23-153):
+//
+//   tc_main <graph> <run> <blocks> [<blockSize> <threadsPerIntersection>]
+//
+// <graph> is a path without extension whose "<graph>.info" text CSR is read
+// (graphRead.cpp:2-26), or one of
+//   gen:er:<n>:<m>:<seed>   gen:rmat:<scale>:<edgefactor>:<seed>
+//   gen:rgg:<n>:<meandeg>:<seed>   <file>.bin (binary CSR)   <file>.graph (METIS)
+// <run> is accepted and ignored as in the reference (param.cu:54-66).
+// With a block size and threads-per-intersection the single run prints, like
+// gpurun.cu:49-51,
+//   nv <tab> total <tab> memAllocMs <tab> tCountMs <tab> execMs
+// where total is the int64 sum of the per-vertex counts (6·T).  The legacy
+// launch shape is validated exactly as param.cu:225-247 does; the tuned
+// two-level kernel runs.  Without a block size it sweeps the compiled
+// kernel variants instead of the reference's 48 legacy shapes and writes
+// "<graph>.o.<blocks>" rows: wg_size items ring allocMs execMs sumMs total.
+// Inputs that are not simple undirected graphs are normalised (self-loops
+// and duplicate arcs dropped, reverse arcs added) with a warning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tc.h"
+#include "tc_host.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+std::vector<std::string> split(const std::string& s, char c) {
+  std::vector<std::string> out;
+  size_t p = 0;
+  for (;;) {
+    size_t q = s.find(c, p);
+    out.push_back(s.substr(p, q == std::string::npos ? std::string::npos : q - p));
+    if (q == std::string::npos) break;
+    p = q + 1;
+  }
+  return out;
+}
+
+bool ends_with(const std::string& s, const char* suf) {
+  size_t n = std::strlen(suf);
+  return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+
+tch_graph* load(const std::string& spec) {
+  if (spec.rfind("gen:", 0) == 0) {
+    auto f = split(spec, ':');
+    if (f.size() == 5 && f[1] == "er")
+      return tch_gen_er(std::atoi(f[2].c_str()), std::atoll(f[3].c_str()), std::strtoull(f[4].c_str(), nullptr, 10));
+    if (f.size() == 5 && f[1] == "rmat") {
+      int scale = std::atoi(f[2].c_str());
+      long long ef = std::atoll(f[3].c_str());
+      return tch_gen_rmat(scale, ef << scale, 0.57, 0.19, 0.19, std::strtoull(f[4].c_str(), nullptr, 10), 0);
+    }
+    if (f.size() == 5 && f[1] == "rgg")
+      return tch_gen_rgg(std::atoi(f[2].c_str()), std::atof(f[3].c_str()), std::strtoull(f[4].c_str(), nullptr, 10));
+    std::fprintf(stderr, "unknown generator spec %s\n", spec.c_str());
+    return nullptr;
+  }
+  if (ends_with(spec, ".bin")) return tch_read_bin(spec.c_str());
+  if (ends_with(spec, ".graph") || ends_with(spec, ".metis")) return tch_read_metis(spec.c_str());
+  return tch_read_info(spec.c_str());
+}
+
+#define HIPCHECK(x)                                                                         \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) {                                                                 \
+      std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+void usage() {
+  std::fprintf(stderr,
+               "usage: tc_main <graph> <run> <blocks> [<blockSize> <threadsPerIntersection>]\n"
+               "  <graph>: path without .info | gen:er:n:m:seed | gen:rmat:scale:ef:seed |\n"
+               "           gen:rgg:n:deg:seed | file.bin | file.graph\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    usage();
+    return 2;
+  }
+  const std::string spec = argv[1];
+  const long long blocks = std::atoll(argv[3]);
+  const bool single = argc > 4;
+  long long bs = 0, tpi = 0;
+  if (blocks <= 0) {
+    std::fprintf(stderr, "Number of blocks not provided for GPU(1) run.\n");
+    return 2;
+  }
+  if (single) {
+    bs = std::atoll(argv[4]);
+    static const long long kBs[] = {32, 64, 96, 128, 192, 256, 512, 1024};
+    static const long long kTpi[] = {1, 2, 4, 8, 16, 32};
+    if (std::find(std::begin(kBs), std::end(kBs), bs) == std::end(kBs)) {
+      std::fprintf(stderr, "Invalid block size parameter(32,64,96,128,192,256,512,1024).\n");
+      return 2;
+    }
+    if (argc < 6) {
+      std::fprintf(stderr, "Threads per intersection not provided for single GPU run.\n");
+      return 2;
+    }
+    tpi = std::atoll(argv[5]);
+    if (std::find(std::begin(kTpi), std::end(kTpi), tpi) == std::end(kTpi)) {
+      std::fprintf(stderr, "Invalid threads per intersection(1,2,4,8,16,32).\n");
+      return 2;
+    }
+  }
+
+  tch_graph* g = load(spec);
+  if (!g) {
+    std::fprintf(stderr, "cannot load %s: %s\n", spec.c_str(), tch_last_error());
+    return 1;
+  }
+  int32_t nv = tch_graph_n(g);
+  int64_t nnz = tch_graph_nnz(g);
+  if (tch_check_simple(nv, nnz, tch_graph_off(g), tch_graph_ind(g)) != TCH_OK) {
+    std::fprintf(stderr, "warning: %s; normalising to a simple undirected graph\n", tch_last_error());
+    tch_graph* h = tch_normalize(g);
+    tch_graph_free(g);
+    if (!h) {
+      std::fprintf(stderr, "normalise failed: %s\n", tch_last_error());
+      return 1;
+    }
+    g = h;
+    nv = tch_graph_n(g);
+    nnz = tch_graph_nnz(g);
+  }
+
+  tc_ctx* ctx = nullptr;
+  if (tc_ctx_create(0, &ctx) != TC_OK) {
+    std::fprintf(stderr, "tc_ctx_create failed\n");
+    return 1;
+  }
+  HIPCHECK(hipFree(nullptr));  // context warm-up, as gpurun.cu:10-12
+
+  auto t0 = clk::now();
+  int32_t *d_off = nullptr, *d_ind = nullptr, *d_out = nullptr;
+  int64_t* d_total = nullptr;
+  HIPCHECK(hipMalloc(&d_off, 4 * ((size_t)nv + 1)));
+  HIPCHECK(hipMalloc(&d_ind, 4 * std::max<size_t>((size_t)nnz, 1)));
+  HIPCHECK(hipMalloc(&d_out, 4 * std::max<size_t>((size_t)nv, 1)));
+  HIPCHECK(hipMalloc(&d_total, 8));
+  HIPCHECK(hipMemcpy(d_off, tch_graph_off(g), 4 * ((size_t)nv + 1), hipMemcpyHostToDevice));
+  if (nnz) HIPCHECK(hipMemcpy(d_ind, tch_graph_ind(g), 4 * (size_t)nnz, hipMemcpyHostToDevice));
+  tc_ctx_reserve(ctx, tc_scratch_bytes(nv, nnz));
+  HIPCHECK(hipDeviceSynchronize());
+  const double alloc_ms = ms_since(t0);
+
+  int rc = 0;
+  if (single) {
+    auto t1 = clk::now();
+    rc = tc_count_i32(ctx, nv, nnz, d_off, d_ind, d_out, d_total, nullptr, nullptr);
+    HIPCHECK(hipDeviceSynchronize());
+    const double count_ms = ms_since(t1);
+    if (rc != TC_OK) {
+      std::fprintf(stderr, "count failed: %s\n", tc_last_error(ctx));
+      return 1;
+    }
+    int64_t total = 0;
+    HIPCHECK(hipMemcpy(&total, d_total, 8, hipMemcpyDeviceToHost));
+    std::printf("%d\t%lld\t%g\t%g\t%g\n", nv, (long long)total, alloc_ms, count_ms, ms_since(t0));
+  } else {
+    std::string base = spec;
+    for (char& ch : base)
+      if (ch == ':' || ch == '/') ch = '_';
+    std::string fname = base + ".o." + std::to_string(blocks);
+    FILE* f = std::fopen(fname.c_str(), "w");
+    if (!f) {
+      std::fprintf(stderr, "cannot write %s\n", fname.c_str());
+      return 1;
+    }
+    static const int kVar[][3] = {{256, 16, 4096}, {256, 8, 2048}, {512, 8, 4096}, {256, 32, 8192}, {128, 32, 4096}};
+    for (const auto& v : kVar) {
+      tc_cfg cfg{};
+      cfg.wg_size = v[0];
+      cfg.items = v[1];
+      cfg.ring = v[2];
+      auto t1 = clk::now();
+      rc = tc_count_i32(ctx, nv, nnz, d_off, d_ind, d_out, d_total, &cfg, nullptr);
+      int64_t total = 0;
+      HIPCHECK(hipMemcpy(&total, d_total, 8, hipMemcpyDeviceToHost));
+      const double ex = ms_since(t1);
+      if (rc != TC_OK) {
+        std::fprintf(stderr, "count failed: %s\n", tc_last_error(ctx));
+        break;
+      }
+      std::fprintf(f, "%d\t%d\t%d\t%g\t%g\t%g\t%lld\n", v[0], v[1], v[2], alloc_ms, ex, ex + alloc_ms,
+                   (long long)total);
+    }
+    std::fclose(f);
+    std::printf("wrote %s\n", fname.c_str());
+  }
+  (void)hipFree(d_off);
+  (void)hipFree(d_ind);
+  (void)hipFree(d_out);
+  (void)hipFree(d_total);
+  tc_ctx_destroy(ctx);
+  tch_graph_free(g);
+  return rc == TC_OK ? 0 : 1;
+}

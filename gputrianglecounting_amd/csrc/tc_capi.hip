@@ -1,0 +1,439 @@
+// C-ABI of the gfx950 triangle counter (declared in include/tc.h).
+//
+// The context owns device scratch for the edge list and work prefix, sized
+// from nv/nnz on the host so nothing is read back from the device: every
+// entry point is stream-ordered and asynchronous except tc_validate_i32.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "tc.h"
+#include "tc_device.h"
+
+struct tc_ctx {
+  int device = 0;
+  int cus = 256;
+  std::string err;
+  void* scratch = nullptr;
+  size_t cap = 0;
+  tc_cfg def{};
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_recorded = false;
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+int set_err(tc_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(tc_ctx* c, hipError_t e, const char* where) {
+  return set_err(c, TC_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+tc_cfg builtin_default() {
+  tc_cfg c{};
+  c.wg_size = 256;
+  c.items = 16;
+  c.ring = 4096;
+  c.small_max = 4096;
+  c.wg_per_cu = 16;
+  c.algo = 0;
+  return c;
+}
+
+// Resolve a configuration (zero fields take the context default) to a
+// compiled kernel variant.  Returns the variant index or -1.
+int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
+  tc_cfg c = ctx->def;
+  if (in) {
+    if (in->wg_size) c.wg_size = in->wg_size;
+    if (in->items) c.items = in->items;
+    if (in->ring) c.ring = in->ring;
+    if (in->small_max) c.small_max = in->small_max;
+    if (in->wg_per_cu) c.wg_per_cu = in->wg_per_cu;
+    c.algo = in->algo;
+  }
+  int vi = -1;
+  for (int i = 0; i < tcd::num_variants(); ++i) {
+    tcd::Variant v = tcd::variant(i);
+    if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring) vi = i;
+  }
+  if (vi < 0 || c.wg_per_cu < 1 || c.wg_per_cu > 256 || c.small_max < 1 || c.algo < 0 || c.algo > 1) return -1;
+  const int lanecap = c.wg_size * c.items;
+  if (c.small_max > lanecap) c.small_max = lanecap;
+  *out = c;
+  return vi;
+}
+
+// Scratch layout shared by the single-graph, batch and total-only paths.
+struct Layout {
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, scan, ooff, oind, total;
+  int64_t ecap, ocap;
+};
+
+Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
+  Layout L{};
+  size_t p = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = p;
+    p += align_up(bytes);
+    return at;
+  };
+  // Undirected edges: at most nnz/2 for a symmetric CSR.  In the oriented
+  // mode every kept arc is an edge, again at most nnz/2.
+  L.ecap = nnz / 2 + 1;
+  L.ocap = orient ? nnz / 2 + 1 : 0;
+  L.voff = take(8 * (size_t)(nv + 1));
+  L.vbase = take(batch ? 4 * (size_t)(nv + 1) : 0);
+  L.ecnt = take(4 * (size_t)(nv + 1));
+  L.eoff = take(8 * (size_t)(nv + 1));
+  L.eu = take(4 * (size_t)L.ecap);
+  L.ew = take(4 * (size_t)L.ecap);
+  L.ework = take(4 * (size_t)L.ecap);
+  L.wpre = take(8 * (size_t)(L.ecap + 1));
+  L.dscal = take(8 * 4);
+  const int64_t ncap = std::max<int64_t>(nv + 1, L.ecap);
+  L.scan = take(8 * tcd::scan_tmp_elems(ncap));
+  L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
+  L.oind = take(orient ? 4 * (size_t)L.ocap : 0);
+  L.total = p;
+  return L;
+}
+
+int ensure(tc_ctx* c, size_t bytes) {
+  if (bytes <= c->cap) return TC_OK;
+  if (c->scratch) {
+    hipError_t e = hipFree(c->scratch);
+    c->scratch = nullptr;
+    c->cap = 0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+  }
+  size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc(&c->scratch, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    c->scratch = nullptr;
+    return set_err(c, TC_ENOMEM, "device scratch of " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
+  }
+  c->cap = want;
+  return TC_OK;
+}
+
+tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
+  char* b = static_cast<char*>(c->scratch);
+  tcd::Prep p{};
+  p.voff = reinterpret_cast<int64_t*>(b + L.voff);
+  p.vbase = batch ? reinterpret_cast<int32_t*>(b + L.vbase) : nullptr;
+  p.ecnt = reinterpret_cast<int32_t*>(b + L.ecnt);
+  p.eoff = reinterpret_cast<int64_t*>(b + L.eoff);
+  p.eu = reinterpret_cast<int32_t*>(b + L.eu);
+  p.ew = reinterpret_cast<int32_t*>(b + L.ew);
+  p.ework = reinterpret_cast<int32_t*>(b + L.ework);
+  p.wpre = reinterpret_cast<int64_t*>(b + L.wpre);
+  p.dscal = reinterpret_cast<int64_t*>(b + L.dscal);
+  p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
+  p.ecap = L.ecap;
+  p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
+  p.oind = L.ocap ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
+  return p;
+}
+
+void record(tc_ctx* c, int i, hipStream_t s) {
+  if (c->timing && c->ev[i]) (void)hipEventRecord(c->ev[i], s);
+}
+
+int count_wg(const tc_ctx* c, const tc_cfg& cfg) {
+  int n = cfg.wg_per_cu * c->cus;
+  if (n >= 64) n &= ~7;
+  return std::max(n, 1);
+}
+
+// Edge list (already built in p) -> counts.
+int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const int32_t* ind, int32_t* out,
+              int64_t* total, int32_t mult, hipStream_t s) {
+  tcd::CountArgs a{};
+  a.wpre = p.wpre;
+  a.eu = p.eu;
+  a.ew = p.ew;
+  a.voff = p.voff;
+  a.ind = ind;
+  a.dscal = p.dscal;
+  a.out = out;
+  a.total = reinterpret_cast<unsigned long long*>(total);
+  a.total_mult = mult;
+  a.small_max = cfg.small_max;
+  a.min_units = 4 * cfg.wg_size * cfg.items;
+  const int nwg = count_wg(c, cfg);
+  a.nwg = nwg;
+  hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, s);
+  if (e != hipSuccess) return hip_fail(c, e, "count kernel launch");
+  return TC_OK;
+}
+
+std::mutex g_shim_mu;
+tc_ctx* g_shim_ctx[64] = {nullptr};
+
+}  // namespace
+
+extern "C" {
+
+int tc_ctx_create(int device, tc_ctx** out) {
+  if (!out) return TC_EINVAL;
+  *out = nullptr;
+  int dev = device;
+  hipError_t e;
+  if (dev < 0) {
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return TC_EHIP;
+  }
+  e = hipSetDevice(dev);
+  if (e != hipSuccess) return TC_EHIP;
+  auto* c = new tc_ctx;
+  c->device = dev;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) c->cus = cus;
+  c->def = builtin_default();
+  for (auto& ev : c->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
+  }
+  *out = c;
+  return TC_OK;
+}
+
+int tc_ctx_destroy(tc_ctx* c) {
+  if (!c) return TC_EINVAL;
+  int dev0 = 0;
+  (void)hipGetDevice(&dev0);
+  (void)hipSetDevice(c->device);
+  if (c->scratch) (void)hipFree(c->scratch);
+  for (auto& ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipSetDevice(dev0);
+  delete c;
+  return TC_OK;
+}
+
+const char* tc_last_error(const tc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t tc_scratch_bytes(int32_t nv, int64_t nnz) { return layout(nv, nnz, false, false).total; }
+size_t tc_batch_scratch_bytes(int32_t G, int64_t nv_total, int64_t nnz_total) {
+  return layout(nv_total, nnz_total, true, false).total;
+}
+
+int tc_ctx_reserve(tc_ctx* c, size_t bytes) {
+  if (!c) return TC_EINVAL;
+  return ensure(c, bytes);
+}
+
+int tc_ctx_set_default_cfg(tc_ctx* c, const tc_cfg* cfg) {
+  if (!c || !cfg) return TC_EINVAL;
+  tc_cfg r;
+  tc_cfg saved = c->def;
+  c->def = builtin_default();
+  if (resolve(c, cfg, &r) < 0) {
+    c->def = saved;
+    return set_err(c, TC_EINVAL, "unsupported configuration");
+  }
+  c->def = r;
+  return TC_OK;
+}
+
+int tc_ctx_get_default_cfg(const tc_ctx* c, tc_cfg* cfg) {
+  if (!c || !cfg) return TC_EINVAL;
+  *cfg = c->def;
+  return TC_OK;
+}
+
+int tc_ctx_set_timing(tc_ctx* c, int enable) {
+  if (!c) return TC_EINVAL;
+  c->timing = enable != 0;
+  c->ev_recorded = false;
+  return TC_OK;
+}
+
+int tc_last_timing(tc_ctx* c, float ms[4]) {
+  if (!c || !ms) return TC_EINVAL;
+  if (!c->timing || !c->ev_recorded) return set_err(c, TC_EINVAL, "no timed call recorded");
+  hipError_t e = hipEventSynchronize(c->ev[3]);
+  if (e != hipSuccess) return hip_fail(c, e, "hipEventSynchronize");
+  if (hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]) != hipSuccess ||
+      hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]) != hipSuccess ||
+      hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]) != hipSuccess ||
+      hipEventElapsedTime(&ms[3], c->ev[0], c->ev[3]) != hipSuccess)
+    return set_err(c, TC_EHIP, "hipEventElapsedTime failed");
+  return TC_OK;
+}
+
+int tc_device_cus(const tc_ctx* c) { return c ? c->cus : 0; }
+
+int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind, int32_t* d_out,
+                 int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || (nv > 0 && !d_out))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv, nnz, false, false);
+  int rc = ensure(c, L.total);
+  if (rc) return rc;
+  tcd::Prep p = make_prep(c, L, false);
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
+  if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "edge list");
+  record(c, 1, s);
+  rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
+  if (rc) return rc;
+  record(c, 2, s);
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
+int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                       int64_t* d_total, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || !d_total) return set_err(c, TC_EINVAL, "invalid argument");
+  if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  tc_cfg cfg;
+  const int vi = resolve(c, nullptr, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv, nnz, false, true);
+  int rc = ensure(c, L.total);
+  if (rc) return rc;
+  tcd::Prep p = make_prep(c, L, false);
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
+  if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+  // The oriented CSR becomes the graph: voff <- ooff, ind <- oind.
+  tcd::Prep q = p;
+  q.voff = p.ooff;
+  if ((e = tcd::launch_build_edges(q, nv, p.ooff, p.oind, 1, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "edge list");
+  record(c, 1, s);
+  rc = run_count(c, q, cfg, vi, p.oind, nullptr, d_total, 6, s);
+  if (rc) return rc;
+  record(c, 2, s);
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
+int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,
+                       const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals,
+                       const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (G <= 0 || !d_tab || !d_off_cat || nv_total < 0 || nnz_total < 0 || (nnz_total > 0 && !d_ind_cat) ||
+      (nv_total > 0 && !d_out_cat))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nv_total >= INT32_MAX) return set_err(c, TC_ERANGE, "total vertex count beyond int32");
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv_total, nnz_total, true, false);
+  int rc = ensure(c, L.total);
+  if (rc) return rc;
+  tcd::Prep p = make_prep(c, L, true);
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i32(d_out_cat, nv_total, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_voff_batch(d_tab, G, d_off_cat, nv_total, p.voff, p.vbase, s)) != hipSuccess)
+    return hip_fail(c, e, "voff");
+  if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "edge list");
+  record(c, 1, s);
+  rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
+  if (rc) return rc;
+  record(c, 2, s);
+  if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
+    return hip_fail(c, e, "totals");
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
+int tc_lcc_f64(tc_ctx* c, int32_t nv, const int32_t* d_off, const int32_t* d_out, double* d_lcc, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || !d_off || (nv > 0 && (!d_out || !d_lcc))) return set_err(c, TC_EINVAL, "invalid argument");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  e = tcd::launch_lcc(d_off, nv, d_out, d_lcc, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TC_OK : hip_fail(c, e, "lcc");
+}
+
+int tc_validate_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind, int* h_status,
+                    void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || !h_status) return set_err(c, TC_EINVAL, "invalid argument");
+  if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  int rc = ensure(c, std::max<size_t>(c->cap, 256));
+  if (rc) return rc;
+  int32_t* flag = static_cast<int32_t*>(c->scratch);
+  if ((e = tcd::launch_fill_i32(flag, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_validate(d_off, d_ind, nv, nnz, flag, c->cus, s)) != hipSuccess) return hip_fail(c, e, "validate");
+  int32_t h = 0;
+  if ((e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return hip_fail(c, e, "copy");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(c, e, "sync");
+  *h_status = h ? TC_EGRAPH : TC_OK;
+  if (h) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "not a simple undirected CSR:%s%s%s%s", (h & 1) ? " ids/offsets out of range;" : "",
+                  (h & 2) ? " row not strictly increasing;" : "", (h & 4) ? " self-loop;" : "",
+                  (h & 8) ? " arc without reverse;" : "");
+    c->err = msg;
+  }
+  return TC_OK;
+}
+
+int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv, const int32_t* d_off,
+                      const int32_t* d_ind, int32_t* outPutTriangles, int32_t threads_per_block, int32_t number_blocks,
+                      int32_t shifter) {
+  static const unsigned kBlockSizes[] = {32, 64, 96, 128, 192, 256, 512, 1024};
+  static const int32_t kTpi[] = {1, 2, 4, 8, 16, 32};
+  bool bs_ok = std::find(std::begin(kBlockSizes), std::end(kBlockSizes), numberThreads) != std::end(kBlockSizes);
+  bool tpi_ok = std::find(std::begin(kTpi), std::end(kTpi), threads_per_block) != std::end(kTpi);
+  if (!bs_ok || !tpi_ok || numberBlocks == 0 || number_blocks != (int32_t)numberThreads / threads_per_block ||
+      (1 << shifter) != threads_per_block || nv < 0 || !d_off)
+    return TC_EINVAL;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TC_EHIP;
+  tc_ctx* c;
+  {
+    std::lock_guard<std::mutex> lk(g_shim_mu);
+    if (!g_shim_ctx[dev]) {
+      if (tc_ctx_create(dev, &g_shim_ctx[dev]) != TC_OK) return TC_EHIP;
+    }
+    c = g_shim_ctx[dev];
+  }
+  // nnz from the device row offsets, as the reference kernel reads them.
+  int32_t nnz = 0;
+  hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return TC_EHIP;
+  return tc_count_i32(c, nv, nnz, d_off, d_ind, outPutTriangles, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Internal interface between the C-ABI layer (tc_capi.hip) and the kernels
+// (tc_kernels.hip).  Not part of the public API.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace tcd {
+
+// Arguments of the count kernels.  The edge list is built on the device:
+// edge e joins global vertices eu[e] < ew[e] (or an oriented arc in the
+// total-only mode); wpre is the exclusive prefix of per-edge work
+// |N(u)| + |N(w)| with wpre[E] = W; dscal[0] = E, dscal[1] = W.
+struct CountArgs {
+  const int64_t* wpre;
+  const int32_t* eu;
+  const int32_t* ew;
+  const int64_t* voff;  // global row offsets [nv + 1]
+  const int32_t* ind;   // neighbour ids (graph-local in batches)
+  const int64_t* dscal;
+  int32_t* out;                 // per-vertex counts (nullable in total-only mode)
+  unsigned long long* total;    // int64 total (nullable)
+  int32_t total_mult;           // total += total_mult * |N(u) ∩ N(w)| per edge
+  int32_t small_max;
+  int32_t nwg;                  // workgroups launched
+  int32_t min_units;            // smallest level-1 range per workgroup
+};
+
+// Kernel variants compiled into the library.
+struct Variant {
+  int wg_size, items, ring;
+};
+int num_variants();
+Variant variant(int i);
+
+// Scratch sizing and prep/count launchers (stream-ordered, no host sync).
+struct Prep {
+  int64_t* voff;    // [nv + 1]
+  int32_t* vbase;   // [nv] graph v_base per vertex, or nullptr (single graph)
+  int32_t* ecnt;    // [nv]
+  int64_t* eoff;    // [nv + 1]
+  int32_t* eu;      // [ecap]
+  int32_t* ew;      // [ecap]
+  int32_t* ework;   // [ecap]
+  int64_t* wpre;    // [ecap + 1]
+  int64_t* dscal;   // [4]
+  int64_t* scan_tmp;
+  int64_t ecap;
+  // oriented copy for the total-only mode
+  int64_t* ooff;    // [nv + 1]
+  int32_t* oind;    // [nnz / 2 + 1]
+};
+
+size_t scan_tmp_elems(int64_t n_cap);
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+hipError_t launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
+hipError_t launch_voff_from_off32(const int32_t* off, int32_t nv, int64_t* voff, hipStream_t s);
+hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_cat, int64_t nv_total,
+                             int64_t* voff, int32_t* vbase, hipStream_t s);
+// mode 0: undirected edges u < w with both degrees >= 2 (per-vertex counts)
+// mode 1: all arcs of an oriented CSR whose endpoints both have out-arcs
+hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
+                              int cus, hipStream_t s);
+hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
+hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, hipStream_t s);
+hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
+hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,
+                           int cus, hipStream_t s);
+hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals,
+                               hipStream_t s);
+
+}  // namespace tcd

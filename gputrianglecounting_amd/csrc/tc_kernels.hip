@@ -1,0 +1,832 @@
+// Device kernels of the gfx950 per-vertex triangle counter.
+//
+// Pipeline for one graph (all stream-ordered, no host synchronisation):
+//   1. voff      int64 row offsets (single graph: widened off; batch: local
+//                offsets of every graph rebased onto one vertex space)
+//   2. edges     for every undirected edge u < w whose endpoints both have
+//                degree >= 2 (the reference skips the others too,
+//                clusteringCount.cu:339), write (u, w) and its Merge Path
+//                length |N(u)| + |N(w)|; exclusive int64 prefix -> wpre
+//   3. count     two-level Merge Path (k_count_mp): level 1 cuts [0, W)
+//                into equal ranges, one per workgroup; level 2 splits every
+//                LDS tile of a range into 64-lane wave diagonals
+//   4. finalise  LCC / per-graph totals when asked
+//
+// The reference kernel (clusteringCount.cu:314-359) gives each block an
+// equal number of vertices (calcWorkPerBlock, 248-264) and intersects both
+// arcs of every edge; here each undirected edge is intersected once and its
+// count added to both endpoints, and the work split is by cumulative
+// intersection length, so a hub's edges spread over many workgroups.
+#include <climits>
+
+#include "tc_device.h"
+
+namespace tcd {
+namespace {
+
+constexpr int kScanBS = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBS * kScanItems;
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int32_t wave_sum32(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum; every thread gets the result.  red must hold BS/64 slots.
+template <int BS>
+__device__ __forceinline__ int64_t block_sum64(int64_t v, int64_t* red) {
+  constexpr int NW = BS / 64;
+  v = wave_sum64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  int64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) s += red[q];
+  __syncthreads();
+  return s;
+}
+
+// Inclusive scan over a wave of 64 lanes.
+__device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_voff_from_off32(const int32_t* __restrict__ off, int64_t n1, int64_t* __restrict__ voff) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x)
+    voff[i] = off[i];
+}
+
+// Batch: global vertex v of graph g (v_base[g] <= v < v_base[g+1]) gets
+// voff[v] = arc_base[g] + off_cat[v_base[g] + g + (v - v_base[g])].
+__global__ void k_voff_batch(const int64_t* __restrict__ tab, int32_t G, const int32_t* __restrict__ off_cat,
+                             int64_t nv, int64_t* __restrict__ voff, int32_t* __restrict__ vbase) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v <= nv; v += (int64_t)gridDim.x * blockDim.x) {
+    if (v == nv) {
+      voff[nv] = tab[2 * G + 1];
+      continue;
+    }
+    int32_t lo = 0, hi = G;  // last g with v_base[g] <= v
+    while (hi - lo > 1) {
+      int32_t mid = (lo + hi) >> 1;
+      if (tab[2 * mid] <= v) lo = mid;
+      else hi = mid;
+    }
+    const int64_t vb = tab[2 * lo];
+    voff[v] = tab[2 * lo + 1] + off_cat[vb + lo + (v - vb)];
+    vbase[v] = (int32_t)vb;
+  }
+}
+
+// ---------------------------------------------------------------- scans
+// Exclusive prefix sum of n elements (n = *n_dev when n_dev != nullptr,
+// else n_host) into out[0..n], out[n] = total.  Three passes: per-tile sums,
+// scan of the tile sums (recursively), per-tile scan with the tile prefix.
+template <typename Tin>
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const Tin* __restrict__ in, const int64_t* n_dev,
+                                                         int64_t n_host, int64_t* __restrict__ bsum) {
+  __shared__ int64_t red[kScanBS / 64];
+  const int64_t n = n_dev ? *n_dev : n_host;
+  const int64_t t0 = (int64_t)blockIdx.x * kScanTile;
+  int64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = t0 + (int64_t)k * kScanBS + threadIdx.x;
+    if (i < n) s += (int64_t)in[i];
+  }
+  s = block_sum64<kScanBS>(s, red);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+template <typename Tin>
+__global__ __launch_bounds__(kScanBS) void k_scan_apply(const Tin* __restrict__ in, const int64_t* n_dev,
+                                                        int64_t n_host, const int64_t* __restrict__ bpre,
+                                                        int64_t* __restrict__ out) {
+  __shared__ int64_t tile[kScanTile];
+  __shared__ int64_t wsum[kScanBS / 64];
+  const int64_t n = n_dev ? *n_dev : n_host;
+  const int64_t t0 = (int64_t)blockIdx.x * kScanTile;
+  if (t0 >= n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;  // n == 0
+    return;
+  }
+  for (int k = 0; k < kScanItems; ++k) {
+    int idx = k * kScanBS + threadIdx.x;
+    int64_t i = t0 + idx;
+    tile[idx] = i < n ? (int64_t)in[i] : 0;
+  }
+  __syncthreads();
+  int64_t run = 0;
+  const int base = threadIdx.x * kScanItems;
+  for (int k = 0; k < kScanItems; ++k) run += tile[base + k];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t inc = wave_incl_scan64(run);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int64_t wpre = 0;
+  for (int q = 0; q < wid; ++q) wpre += wsum[q];
+  int64_t acc = (bpre ? bpre[blockIdx.x] : 0) + wpre + inc - run;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t v = tile[base + k];
+    int64_t i = t0 + base + k;
+    if (i < n) out[i] = acc;
+    if (i == n - 1) out[n] = acc + v;
+    acc += v;
+  }
+}
+
+template <typename Tin>
+hipError_t scan_excl(const Tin* in, const int64_t* n_dev, int64_t n_cap, int64_t* out, int64_t* tmp,
+                     hipStream_t s) {
+  int64_t nb = (n_cap + kScanTile - 1) / kScanTile;
+  if (nb < 1) nb = 1;
+  int64_t* bsum = tmp;
+  int64_t* bpre = tmp + nb;
+  int64_t* rest = bpre + nb + 1;
+  if (nb > 1) {
+    hipLaunchKernelGGL(k_scan_reduce<Tin>, dim3((unsigned)nb), dim3(kScanBS), 0, s, in, n_dev, n_cap, bsum);
+    hipError_t e = scan_excl<int64_t>(bsum, nullptr, nb, bpre, rest, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_apply<Tin>, dim3((unsigned)nb), dim3(kScanBS), 0, s, in, n_dev, n_cap,
+                       (const int64_t*)bpre, out);
+  } else {
+    hipLaunchKernelGGL(k_scan_apply<Tin>, dim3(1), dim3(kScanBS), 0, s, in, n_dev, n_cap,
+                       (const int64_t*)nullptr, out);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- edge list
+// One wave per vertex.  mode 0: count neighbours w > v (graph-local ids)
+// with d(v) >= 2 and d(w) >= 2.  mode 1 (oriented CSR): count neighbours w
+// with d+(v) >= 1 and d+(w) >= 1.
+__device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
+  return mode == 0 ? (wl > vl && dv >= 2 && dw >= 2) : (dv >= 1 && dw >= 1);
+}
+
+__global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                    const int32_t* __restrict__ vbase, int64_t nv, int mode,
+                                                    int32_t* __restrict__ ecnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+    const int32_t vb = vbase ? vbase[v] : 0;
+    const int32_t vl = (int32_t)(v - vb);
+    int32_t c = 0;
+    if (dv >= (mode == 0 ? 2 : 1)) {
+      for (int64_t k = b + lane; k < e; k += 64) {
+        const int32_t wl = ind[k];
+        const int64_t wg = (int64_t)vb + wl;
+        const int64_t dw = voff[wg + 1] - voff[wg];
+        c += edge_keep(mode, vl, wl, dv, dw);
+      }
+    }
+    c = wave_sum32(c);
+    if (lane == 0) ecnt[v] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                   const int32_t* __restrict__ vbase, int64_t nv, int mode,
+                                                   const int64_t* __restrict__ eoff, int64_t ecap,
+                                                   int32_t* __restrict__ eu, int32_t* __restrict__ ew,
+                                                   int32_t* __restrict__ ework) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+    if (dv < (mode == 0 ? 2 : 1)) continue;
+    const int32_t vb = vbase ? vbase[v] : 0;
+    const int32_t vl = (int32_t)(v - vb);
+    int64_t pos = eoff[v];
+    for (int64_t k0 = b; k0 < e; k0 += 64) {
+      const int64_t k = k0 + lane;
+      bool keep = false;
+      int64_t wg = 0, dw = 0;
+      if (k < e) {
+        const int32_t wl = ind[k];
+        wg = (int64_t)vb + wl;
+        dw = voff[wg + 1] - voff[wg];
+        keep = edge_keep(mode, vl, wl, dv, dw);
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int64_t idx = pos + __popcll(m & lt);
+        if (idx < ecap) {
+          eu[idx] = (int32_t)v;
+          ew[idx] = (int32_t)wg;
+          ework[idx] = (int32_t)(dv + dw);
+        }
+      }
+      pos += __popcll(m);
+    }
+  }
+}
+
+__global__ void k_edge_finish(const int64_t* __restrict__ eoff, int64_t nv, int64_t ecap, int64_t* __restrict__ dscal) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int64_t E = eoff[nv];
+    dscal[2] = E > ecap ? 1 : 0;  // overflow flag: input was not symmetric
+    dscal[0] = E > ecap ? ecap : E;
+  }
+}
+__global__ void k_work_finish(const int64_t* __restrict__ wpre, int64_t* __restrict__ dscal) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) dscal[1] = wpre[dscal[0]];
+}
+
+// Oriented copy for the total-only mode: keep arc v -> w iff
+// (d(v), v) < (d(w), w); rows stay sorted.
+__device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, int64_t w) {
+  return dv < dw || (dv == dw && v < w);
+}
+__global__ __launch_bounds__(256) void k_orient_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                      int64_t nv, int32_t* __restrict__ ocnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+    int32_t c = 0;
+    for (int64_t k = b + lane; k < e; k += 64) {
+      const int32_t w = ind[k];
+      c += orient_keep(dv, v, voff[w + 1] - voff[w], w);
+    }
+    c = wave_sum32(c);
+    if (lane == 0) ocnt[v] = c;
+  }
+}
+__global__ __launch_bounds__(256) void k_orient_fill(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                     int64_t nv, const int64_t* __restrict__ ooff,
+                                                     int32_t* __restrict__ oind) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+    int64_t pos = ooff[v];
+    for (int64_t k0 = b; k0 < e; k0 += 64) {
+      const int64_t k = k0 + lane;
+      bool keep = false;
+      int32_t w = 0;
+      if (k < e) {
+        w = ind[k];
+        keep = orient_keep(dv, v, voff[w + 1] - voff[w], w);
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) oind[pos + __popcll(m & lt)] = w;
+      pos += __popcll(m);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- count
+// Level-1 range of logical workgroup lg out of nr: [R0, R1) of [0, W).
+__device__ __forceinline__ void level1_range(int64_t W, int nr, int lg, int64_t& R0, int64_t& R1) {
+  const int64_t q = W / nr, r = W % nr;
+  R0 = q * lg + (lg < r ? lg : r);
+  R1 = R0 + q + (lg < r ? 1 : 0);
+}
+
+__device__ __forceinline__ int level1_parts(int64_t W, int nwg, int min_units) {
+  int64_t nr = W / (min_units > 0 ? min_units : 1);
+  if (nr < 1) nr = 1;
+  if (nr > nwg) nr = nwg;
+  return (int)nr;
+}
+
+// Two-level Merge Path count.
+//   BS  threads per workgroup, K merge steps per lane per tile,
+//   P   LDS ring length per list (power of two), M max edges per small tile.
+// A workgroup walks its range of the concatenated Merge Paths of all edges
+// tile by tile.  Ring tiles (one long edge, or a partial edge at a range
+// boundary) stream both lists through two LDS rings so every element is
+// loaded from memory once per pass; small tiles pack up to M whole short
+// edges, each staged as [N(u) | INT_MAX | N(w) | INT_MAX].  In both, lane l
+// takes diagonals [l*K, l*K + K) of the tile, finds its start with a
+// binary search on the staged lists and merges serially.  A match
+// A[x] == B[y] is counted by the lane that consumes A[x] (ties go to A, so
+// B[y] is then the head of B): every match is counted exactly once, with no
+// hand-off between lanes or workgroups.
+template <int BS, int K, int P, int M>
+__global__ __launch_bounds__(BS) void k_count_mp(CountArgs a) {
+  constexpr int NW = BS / 64;
+  constexpr int LANECAP = BS * K;
+  constexpr int RMASK = P - 1;
+  constexpr int TMAX = P - 2;
+  static_assert((P & (P - 1)) == 0, "ring length must be a power of two");
+  static_assert(LANECAP >= TMAX, "lanes must cover a ring tile");
+  static_assert(LANECAP + 2 * M <= 2 * P, "a small tile must fit the LDS buffer");
+  static_assert(M <= BS, "one metadata slot per thread");
+
+  __shared__ int32_t buf[2 * P];
+  __shared__ int32_t m_ws[M + 1];
+  __shared__ int32_t m_la[M], m_lb[M], m_cnt[M], m_u[M];
+  __shared__ int64_t m_ag[M], m_bg[M];
+  __shared__ int64_t red64[NW];
+  __shared__ int32_t red32[NW];
+  __shared__ int32_t s_ij[2];
+  __shared__ int64_t s_e;
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t E = a.dscal[0], W = a.dscal[1];
+  const int nr = level1_parts(W, a.nwg, a.min_units);
+  int lg = blockIdx.x;
+  if (nr == a.nwg && (a.nwg & 7) == 0) lg = (blockIdx.x & 7) * (a.nwg >> 3) + (blockIdx.x >> 3);
+  if (lg >= nr) return;
+  int64_t R0, R1;
+  level1_range(W, nr, lg, R0, R1);
+  if (R0 >= R1) return;
+
+  if (tid == 0) {
+    int64_t lo = 0, hi = E + 1;  // first index with wpre > R0
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (a.wpre[mid] <= R0) lo = mid + 1;
+      else hi = mid;
+    }
+    const int64_t e = lo - 1;
+    const int64_t t = R0 - a.wpre[e];
+    int32_t i = 0;
+    if (t > 0) {
+      const int32_t u = a.eu[e], w = a.ew[e];
+      const int64_t ab = a.voff[u], bb = a.voff[w];
+      const int64_t la = a.voff[u + 1] - ab, lb = a.voff[w + 1] - bb;
+      int64_t l2 = t - lb > 0 ? t - lb : 0, h2 = t < la ? t : la;
+      while (l2 < h2) {
+        int64_t mid = (l2 + h2) >> 1;
+        if (a.ind[ab + mid] <= a.ind[bb + t - mid - 1]) l2 = mid + 1;
+        else h2 = mid;
+      }
+      i = (int32_t)l2;
+    }
+    s_e = e;
+    s_ij[0] = i;
+    s_ij[1] = (int32_t)(t - i);
+  }
+  __syncthreads();
+  int64_t e = s_e;
+  int32_t ci = s_ij[0], cj = s_ij[1];
+  int64_t pos = R0;
+  int64_t my_sum = 0;  // sum of edge counts flushed by this thread
+  __syncthreads();
+
+  while (pos < R1) {
+    const int64_t we0 = a.wpre[e], we1 = a.wpre[e + 1];
+    const int64_t L = we1 - we0;
+    const int64_t done = (int64_t)ci + cj;
+    const int64_t rem = L - done;
+    const bool ring = done > 0 || rem > a.small_max || (R1 - pos) < rem;
+    if (ring) {
+      // ---------------- one edge through the LDS rings
+      const int64_t units = rem < (R1 - pos) ? rem : (R1 - pos);
+      const int32_t u = a.eu[e], w = a.ew[e];
+      const int64_t ab = a.voff[u], bb = a.voff[w];
+      const int32_t la = (int32_t)(a.voff[u + 1] - ab), lb = (int32_t)(a.voff[w + 1] - bb);
+      int32_t a_hi = ci, b_hi = cj;
+      int32_t cnt = 0;
+      int64_t left = units;
+      while (left > 0) {
+        const int32_t T = (int32_t)(left < TMAX ? left : TMAX);
+        const int32_t a_to = ci + T < la ? ci + T : la;
+        const int32_t b_to = cj + T < lb ? cj + T : lb;
+        for (int32_t x = (a_hi > ci ? a_hi : ci) + tid; x < a_to; x += BS) buf[x & RMASK] = a.ind[ab + x];
+        for (int32_t y = (b_hi > cj ? b_hi : cj) + tid; y < b_to; y += BS) buf[P + (y & RMASK)] = a.ind[bb + y];
+        if (tid == 0) {
+          if (a_to == la) buf[la & RMASK] = INT_MAX;
+          if (b_to == lb) buf[P + (lb & RMASK)] = INT_MAX;
+        }
+        a_hi = a_hi > a_to ? a_hi : a_to;
+        b_hi = b_hi > b_to ? b_hi : b_to;
+        __syncthreads();
+        const int32_t d = tid * K;
+        if (d < T) {
+          int32_t l2 = d - (lb - cj);
+          l2 = l2 > 0 ? l2 : 0;
+          int32_t h2 = d < la - ci ? d : la - ci;
+          while (l2 < h2) {
+            const int32_t mid = (l2 + h2) >> 1;
+            const int32_t av = buf[(ci + mid) & RMASK];
+            const int32_t bv = buf[P + ((cj + d - mid - 1) & RMASK)];
+            if (av <= bv) l2 = mid + 1;
+            else h2 = mid;
+          }
+          int32_t ia = ci + l2, jb = cj + d - l2;
+          int32_t av = buf[ia & RMASK], bv = buf[P + (jb & RMASK)];
+          const int32_t steps = K < T - d ? K : T - d;
+          if (steps == K) {
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+              const bool ta = av <= bv;
+              cnt += (av == bv);
+              ia += ta;
+              jb += !ta;
+              const int32_t nvv = buf[ta ? (ia & RMASK) : (P + (jb & RMASK))];
+              av = ta ? nvv : av;
+              bv = ta ? bv : nvv;
+            }
+          } else {
+            for (int s = 0; s < steps; ++s) {
+              const bool ta = av <= bv;
+              cnt += (av == bv);
+              ia += ta;
+              jb += !ta;
+              const int32_t nvv = buf[ta ? (ia & RMASK) : (P + (jb & RMASK))];
+              av = ta ? nvv : av;
+              bv = ta ? bv : nvv;
+            }
+          }
+          if (d + steps == T) {
+            s_ij[0] = ia;
+            s_ij[1] = jb;
+          }
+        }
+        __syncthreads();
+        ci = s_ij[0];
+        cj = s_ij[1];
+        left -= T;
+      }
+      const int64_t c = block_sum64<BS>(cnt, red64);
+      if (tid == 0 && c) {
+        if (a.out) {
+          atomicAdd(&a.out[u], (int32_t)c);
+          atomicAdd(&a.out[w], (int32_t)c);
+        }
+        my_sum += c;
+      }
+      pos += units;
+      if (units == rem) {
+        ++e;
+        ci = 0;
+        cj = 0;
+      }
+    } else {
+      // ---------------- several whole short edges
+      bool ok = false;
+      const int64_t ee = e + tid;
+      int32_t cw = 0, wk = 0;
+      if (tid < M && ee < E) {
+        const int64_t w1 = a.wpre[ee + 1];
+        const int64_t cw64 = w1 - we0;
+        const int64_t wk64 = w1 - a.wpre[ee];
+        ok = cw64 <= LANECAP && cw64 + 2 * (tid + 1) <= 2 * P && wk64 <= a.small_max && w1 <= R1;
+        cw = (int32_t)(cw64 < INT_MAX ? cw64 : INT_MAX);
+        wk = (int32_t)(wk64 < INT_MAX ? wk64 : INT_MAX);
+      }
+      const uint64_t bad = __ballot(!ok);
+      if (lane == 0) red32[wid] = bad ? (wid * 64 + __builtin_ctzll(bad)) : BS;
+      __syncthreads();
+      int32_t c = BS;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) c = red32[q] < c ? red32[q] : c;
+      if (tid < c) {
+        const int32_t u = a.eu[ee], w = a.ew[ee];
+        const int64_t ag = a.voff[u], bg = a.voff[w];
+        m_ws[tid] = cw - wk;
+        m_la[tid] = (int32_t)(a.voff[u + 1] - ag);
+        m_lb[tid] = (int32_t)(a.voff[w + 1] - bg);
+        m_cnt[tid] = 0;
+        m_u[tid] = u;
+        m_ag[tid] = ag;
+        m_bg[tid] = bg;
+        if (tid == c - 1) m_ws[c] = cw;
+      }
+      __syncthreads();
+      const int32_t cum = m_ws[c];
+      {
+        // stage with a power-of-two group of threads per edge
+        const int32_t per = BS / c;
+        const int32_t gsh = 31 - __builtin_clz((unsigned)per);
+        const int32_t k = tid >> gsh, r = tid & ((1 << gsh) - 1), g = 1 << gsh;
+        if (k < c) {
+          const int32_t la = m_la[k], lb = m_lb[k], base = m_ws[k] + 2 * k;
+          const int64_t ag = m_ag[k], bg = m_bg[k];
+          for (int32_t p = r; p < la; p += g) buf[base + p] = a.ind[ag + p];
+          for (int32_t p = r; p < lb; p += g) buf[base + la + 1 + p] = a.ind[bg + p];
+          if (r == 0) {
+            buf[base + la] = INT_MAX;
+            buf[base + la + 1 + lb] = INT_MAX;
+          }
+        }
+      }
+      __syncthreads();
+      const int32_t d0 = tid * K;
+      if (d0 < cum) {
+        int32_t l2 = 0, h2 = c;  // first k with m_ws[k] > d0
+        while (l2 < h2) {
+          const int32_t mid = (l2 + h2) >> 1;
+          if (m_ws[mid] <= d0) l2 = mid + 1;
+          else h2 = mid;
+        }
+        int32_t k = l2 - 1;
+        int32_t left = K < cum - d0 ? K : cum - d0;
+        int32_t d = d0 - m_ws[k];
+        int32_t la = m_la[k], lb = m_lb[k];
+        int32_t abase = m_ws[k] + 2 * k, bbase = abase + la + 1;
+        int32_t x0 = d - lb > 0 ? d - lb : 0, x1 = d < la ? d : la;
+        while (x0 < x1) {
+          const int32_t mid = (x0 + x1) >> 1;
+          if (buf[abase + mid] <= buf[bbase + d - mid - 1]) x0 = mid + 1;
+          else x1 = mid;
+        }
+        int32_t ia = x0, jb = d - x0;
+        int32_t av = buf[abase + ia], bv = buf[bbase + jb];
+        int32_t cnt = 0;
+        for (;;) {
+          const int32_t room = la + lb - d;
+          const int32_t n = left < room ? left : room;
+          for (int32_t s = 0; s < n; ++s) {
+            const bool ta = av <= bv;
+            cnt += (av == bv);
+            ia += ta;
+            jb += !ta;
+            const int32_t nvv = buf[ta ? abase + ia : bbase + jb];
+            av = ta ? nvv : av;
+            bv = ta ? bv : nvv;
+          }
+          left -= n;
+          d += n;
+          if (d == la + lb) {
+            if (cnt) atomicAdd(&m_cnt[k], cnt);
+            cnt = 0;
+            if (left == 0) break;
+            ++k;
+            la = m_la[k];
+            lb = m_lb[k];
+            abase = m_ws[k] + 2 * k;
+            bbase = abase + la + 1;
+            d = 0;
+            ia = 0;
+            jb = 0;
+            av = buf[abase];
+            bv = buf[bbase];
+          } else {
+            break;
+          }
+        }
+        if (cnt) atomicAdd(&m_cnt[k], cnt);
+      }
+      __syncthreads();
+      if (tid < c) {
+        const int32_t cc = m_cnt[tid];
+        if (cc) {
+          if (a.out) atomicAdd(&a.out[a.ew[ee]], cc);
+          my_sum += cc;
+        }
+        // u side: the first edge of each run of equal u adds the run's sum
+        const int32_t u = m_u[tid];
+        if (a.out && (tid == 0 || m_u[tid - 1] != u)) {
+          int32_t s = 0;
+          for (int32_t t2 = tid; t2 < c && m_u[t2] == u; ++t2) s += m_cnt[t2];
+          if (s) atomicAdd(&a.out[u], s);
+        }
+      }
+      __syncthreads();
+      pos += cum;
+      e += c;
+      ci = 0;
+      cj = 0;
+    }
+  }
+  const int64_t tsum = block_sum64<BS>(my_sum, red64);
+  if (tid == 0 && a.total && tsum) atomicAdd(a.total, (unsigned long long)(tsum * a.total_mult));
+}
+
+// One lane per edge, two-pointer merge from global memory.  Simple and
+// unbalanced; kept as an independent cross-check of k_count_mp.
+__global__ __launch_bounds__(256) void k_count_simple(CountArgs a) {
+  __shared__ int64_t red[4];
+  const int64_t E = a.dscal[0];
+  int64_t my = 0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t u = a.eu[e], w = a.ew[e];
+    const int32_t* A = a.ind + a.voff[u];
+    const int32_t* B = a.ind + a.voff[w];
+    const int64_t la = a.voff[u + 1] - a.voff[u], lb = a.voff[w + 1] - a.voff[w];
+    int64_t i = 0, j = 0;
+    int32_t c = 0;
+    while (i < la && j < lb) {
+      const int32_t x = A[i], y = B[j];
+      c += x == y;
+      i += x <= y;
+      j += y <= x;
+    }
+    if (c) {
+      if (a.out) {
+        atomicAdd(&a.out[u], c);
+        atomicAdd(&a.out[w], c);
+      }
+      my += c;
+    }
+  }
+  const int64_t s = block_sum64<256>(my, red);
+  if (threadIdx.x == 0 && a.total && s) atomicAdd(a.total, (unsigned long long)(s * a.total_mult));
+}
+
+// ---------------------------------------------------------------- finalise
+__global__ void k_lcc(const int32_t* __restrict__ off, int32_t nv, const int32_t* __restrict__ out,
+                      double* __restrict__ lcc) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = (int64_t)off[v + 1] - off[v];
+    lcc[v] = d < 2 ? 0.0 : (double)out[v] / (double)(d * (d - 1));
+  }
+}
+
+// Per-graph totals of a batch: one block per graph sums its vertex range.
+__global__ __launch_bounds__(256) void k_graph_totals(const int64_t* __restrict__ tab, int32_t G,
+                                                      const int32_t* __restrict__ out, int64_t* __restrict__ totals) {
+  __shared__ int64_t red[4];
+  for (int32_t g = blockIdx.x; g < G; g += gridDim.x) {
+    const int64_t b = tab[2 * g], e = tab[2 * g + 2];
+    int64_t s = 0;
+    for (int64_t v = b + threadIdx.x; v < e; v += blockDim.x) s += out[v];
+    s = block_sum64<256>(s, red);
+    if (threadIdx.x == 0) totals[g] = s;
+  }
+}
+
+// Simple-graph contract check, one wave per vertex; flag bits:
+// 1 range/offsets, 2 unsorted or duplicate, 4 self-loop, 8 missing reverse.
+__global__ __launch_bounds__(256) void k_validate(const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
+                                                  int32_t nv, int64_t nnz, int32_t* __restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  if (wave == 0 && lane == 0 && (off[0] != 0 || off[nv] != nnz)) atomicOr(flag, 1);
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int64_t b = off[v], e = off[v + 1];
+    if (e < b || b < 0 || e > nnz) {
+      if (lane == 0) atomicOr(flag, 1);
+      continue;
+    }
+    int32_t f = 0;
+    for (int64_t k = b + lane; k < e; k += 64) {
+      const int32_t w = ind[k];
+      if (w < 0 || w >= nv) {
+        f |= 1;
+        continue;
+      }
+      if (k > b && ind[k - 1] >= w) f |= 2;
+      if (w == v) f |= 4;
+      int64_t lo = off[w], hi = off[w + 1];
+      if (lo < 0 || hi > nnz || hi < lo) {
+        f |= 1;
+        continue;
+      }
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (ind[mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo >= off[w + 1] || ind[lo] != v) f |= 8;
+    }
+    if (f) atomicOr(flag, f);
+  }
+}
+
+unsigned grid_for(int64_t n, int bs, int64_t cap = 65536) {
+  int64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// Compiled variants of k_count_mp: {BS, K, P, M}.
+struct VariantDef {
+  int bs, k, p, m;
+  void (*fn)(CountArgs);
+};
+const VariantDef kVariants[] = {
+    {256, 16, 4096, 128, k_count_mp<256, 16, 4096, 128>},
+    {256, 8, 2048, 128, k_count_mp<256, 8, 2048, 128>},
+    {512, 8, 4096, 128, k_count_mp<512, 8, 4096, 128>},
+    {256, 32, 8192, 128, k_count_mp<256, 32, 8192, 128>},
+    {128, 32, 4096, 64, k_count_mp<128, 32, 4096, 64>},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+}  // namespace
+
+int num_variants() { return kNumVariants; }
+Variant variant(int i) { return Variant{kVariants[i].bs, kVariants[i].k, kVariants[i].p}; }
+
+size_t scan_tmp_elems(int64_t n_cap) {
+  size_t total = 0;
+  int64_t n = n_cap;
+  do {
+    int64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb < 1) nb = 1;
+    total += 2 * (size_t)nb + 1;
+    n = nb;
+  } while (n > 1);
+  return total + 8;
+}
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_voff_from_off32(const int32_t* off, int32_t nv, int64_t* voff, hipStream_t s) {
+  hipLaunchKernelGGL(k_voff_from_off32, dim3(grid_for((int64_t)nv + 1, 256, 16384)), dim3(256), 0, s, off,
+                     (int64_t)nv + 1, voff);
+  return hipGetLastError();
+}
+hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_cat, int64_t nv_total, int64_t* voff,
+                             int32_t* vbase, hipStream_t s) {
+  hipLaunchKernelGGL(k_voff_batch, dim3(grid_for(nv_total + 1, 256, 16384)), dim3(256), 0, s, tab, G, off_cat,
+                     nv_total, voff, vbase);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode, int cus,
+                              hipStream_t s) {
+  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 32);
+  if (nv > 0)
+    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode, p.ecnt);
+  hipError_t e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.eoff, p.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  if (nv > 0)
+    hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
+                       (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework);
+  hipLaunchKernelGGL(k_edge_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.eoff, nv, p.ecap, p.dscal);
+  e = scan_excl<int32_t>(p.ework, p.dscal, p.ecap, p.wpre, p.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_work_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.wpre, p.dscal);
+  return hipGetLastError();
+}
+
+hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
+  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 32);
+  if (nv > 0) hipLaunchKernelGGL(k_orient_count, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, nv, p.ecnt);
+  hipError_t e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.ooff, p.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  if (nv > 0)
+    hipLaunchKernelGGL(k_orient_fill, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, nv,
+                       (const int64_t*)p.ooff, p.oind);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, hipStream_t s) {
+  if (algo == 1) {
+    hipLaunchKernelGGL(k_count_simple, dim3(nwg), dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
+  if (vi < 0 || vi >= kNumVariants) return hipErrorInvalidValue;
+  CountArgs b = a;
+  b.nwg = nwg;
+  hipLaunchKernelGGL(kVariants[vi].fn, dim3(nwg), dim3(kVariants[vi].bs), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s) {
+  if (nv <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lcc, dim3(grid_for(nv, 256, 16384)), dim3(256), 0, s, off, nv, out, lcc);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag, int cus,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_validate, dim3(grid_for((int64_t)nv * 64 + 64, 256, (int64_t)cus * 32)), dim3(256), 0, s,
+                     off, ind, nv, nnz, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals, hipStream_t s) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_graph_totals, dim3(G < 65536 ? G : 65536), dim3(256), 0, s, tab, G, out, totals);
+  return hipGetLastError();
+}
+
+}  // namespace tcd

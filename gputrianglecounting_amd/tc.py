@@ -1,0 +1,284 @@
+"""ctypes binding of the gfx950 counter (``libtc.so``, include/tc.h).
+
+Device memory and streams come from PyTorch (plumbing only); every count
+runs in the HIP kernels of ``libtc.so``.  There is no CPU fallback: if the
+library is missing or no GPU is visible, the calls raise.
+
+Reference interface mirrored here:
+  kernelCall<int32_t>(blocks, bs, nv, d_off, d_ind, out, tpi, groups, shifter)
+      src/clusteringCount.cu:415-467            -> :func:`kernel_call`
+  singleParamTestGPURun<T> (H2D, count, total)  src/gpurun.cu:2-56
+                                                -> :meth:`TriangleCounter.count_graph`
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .graph import LIB_DIR, Batch, Graph
+
+TC_OK, TC_EINVAL, TC_ERANGE, TC_EHIP, TC_ENOMEM, TC_EGRAPH = 0, -1, -2, -3, -4, -5
+
+_lib = None
+
+
+class TCError(RuntimeError):
+    def __init__(self, msg: str, code: int):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class TcCfg(C.Structure):
+    """Mirror of ``tc_cfg`` (include/tc.h)."""
+
+    _fields_ = [("wg_size", C.c_int32), ("items", C.c_int32), ("ring", C.c_int32),
+                ("small_max", C.c_int32), ("wg_per_cu", C.c_int32), ("algo", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+    @classmethod
+    def make(cls, **kw) -> "TcCfg":
+        c = cls()
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        return c
+
+
+def lib() -> C.CDLL:
+    """Load ``libtc.so``; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(LIB_DIR, "libtc.so")
+    if not os.path.exists(path):
+        raise TCError(f"{path} is missing: the HIP extension is not built", TC_EINVAL)
+    L = C.CDLL(path)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    sig = {
+        "tc_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "tc_ctx_destroy": (C.c_int, [vp]),
+        "tc_last_error": (C.c_char_p, [vp]),
+        "tc_scratch_bytes": (C.c_size_t, [i32, i64]),
+        "tc_batch_scratch_bytes": (C.c_size_t, [i32, i64, i64]),
+        "tc_ctx_reserve": (C.c_int, [vp, C.c_size_t]),
+        "tc_ctx_set_default_cfg": (C.c_int, [vp, C.POINTER(TcCfg)]),
+        "tc_ctx_get_default_cfg": (C.c_int, [vp, C.POINTER(TcCfg)]),
+        "tc_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+        "tc_last_timing": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "tc_count_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_kernelCall_i32": (C.c_int, [C.c_uint, C.c_uint, i32, vp, vp, vp, i32, i32, i32]),
+        "tc_lcc_f64": (C.c_int, [vp, i32, vp, vp, vp, vp]),
+        "tc_validate_i32": (C.c_int, [vp, i32, i64, vp, vp, C.POINTER(C.c_int), vp]),
+        "tc_batch_count_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_count_total_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "tc_device_cus": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+# Every symbol include/tc.h declares (checked by the CPU test suite).
+EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
+           "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
+           "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
+           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus"]
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise TCError("no GPU visible: the HIP counter has no CPU fallback", TC_EHIP)
+    return torch
+
+
+def _p(t) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        return C.c_void_p(int(_torch().cuda.current_stream().cuda_stream))
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(int(stream.cuda_stream))
+
+
+def _cfgp(cfg):
+    if cfg is None:
+        return None
+    if isinstance(cfg, dict):
+        cfg = TcCfg.make(**cfg)
+    return C.byref(cfg)
+
+
+@dataclass
+class Timing:
+    prep_ms: float
+    count_ms: float
+    finish_ms: float
+    total_ms: float
+
+
+class TriangleCounter:
+    """A ``tc_ctx`` bound to one GPU."""
+
+    def __init__(self, device: int = 0):
+        torch = _torch()
+        self.device = device
+        self._torch = torch
+        with torch.cuda.device(device):
+            h = C.c_void_p()
+            rc = lib().tc_ctx_create(device, C.byref(h))
+            if rc != TC_OK:
+                raise TCError("tc_ctx_create failed", rc)
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().tc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != TC_OK:
+            raise TCError(f"{what}: {lib().tc_last_error(self._h).decode()}", rc)
+
+    @property
+    def cus(self) -> int:
+        return int(lib().tc_device_cus(self._h))
+
+    # ------------------------------------------------------------ config
+    def default_cfg(self) -> dict:
+        c = TcCfg()
+        self._check(lib().tc_ctx_get_default_cfg(self._h, C.byref(c)), "get cfg")
+        return c.as_dict()
+
+    def set_default_cfg(self, **kw) -> None:
+        self._check(lib().tc_ctx_set_default_cfg(self._h, C.byref(TcCfg.make(**kw))), "set cfg")
+
+    def set_timing(self, on: bool = True) -> None:
+        self._check(lib().tc_ctx_set_timing(self._h, 1 if on else 0), "timing")
+
+    def last_timing(self) -> Timing:
+        ms = (C.c_float * 4)()
+        self._check(lib().tc_last_timing(self._h, ms), "timing")
+        return Timing(*[float(x) for x in ms])
+
+    def reserve(self, nbytes: int) -> None:
+        self._check(lib().tc_ctx_reserve(self._h, nbytes), "reserve")
+
+    # ------------------------------------------------------------ device API
+    def count(self, off, ind, out=None, total=None, cfg=None, stream=None):
+        """Per-vertex counts of a device CSR (int32 torch tensors).
+
+        Returns ``(out, total)``: ``out`` int32[nv] = 2·T(v), ``total`` int64[1]."""
+        torch = self._torch
+        nv = int(off.shape[0]) - 1
+        nnz = int(ind.shape[0])
+        if off.dtype != torch.int32 or ind.dtype != torch.int32:
+            raise TypeError("off and ind must be int32")
+        if out is None:
+            out = torch.empty(max(nv, 0), dtype=torch.int32, device=off.device)
+        if total is None:
+            total = torch.empty(1, dtype=torch.int64, device=off.device)
+        rc = lib().tc_count_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(out) if nv else None,
+                                _p(total), _cfgp(cfg), _stream(stream))
+        self._check(rc, "tc_count_i32")
+        return out, total
+
+    def count_total(self, off, ind, total=None, stream=None):
+        """Global-count mode: int64[1] = 6·T (degree-ordered orientation)."""
+        torch = self._torch
+        nv = int(off.shape[0]) - 1
+        nnz = int(ind.shape[0])
+        if total is None:
+            total = torch.empty(1, dtype=torch.int64, device=off.device)
+        rc = lib().tc_count_total_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(total),
+                                      _stream(stream))
+        self._check(rc, "tc_count_total_i32")
+        return total
+
+    def lcc(self, off, out, lcc=None, stream=None):
+        torch = self._torch
+        nv = int(off.shape[0]) - 1
+        if lcc is None:
+            lcc = torch.empty(max(nv, 0), dtype=torch.float64, device=off.device)
+        rc = lib().tc_lcc_f64(self._h, nv, _p(off), _p(out) if nv else None, _p(lcc) if nv else None,
+                              _stream(stream))
+        self._check(rc, "tc_lcc_f64")
+        return lcc
+
+    def validate(self, off, ind, stream=None) -> int:
+        """0 if the device CSR is a simple undirected graph, else TC_EGRAPH."""
+        st = C.c_int(-99)
+        nv = int(off.shape[0]) - 1
+        rc = lib().tc_validate_i32(self._h, nv, int(ind.shape[0]), _p(off), _p(ind) if ind.numel() else None,
+                                   C.byref(st), _stream(stream))
+        self._check(rc, "tc_validate_i32")
+        return int(st.value)
+
+    def last_error(self) -> str:
+        return lib().tc_last_error(self._h).decode()
+
+    def batch_count(self, tab, off_cat, ind_cat, nv_total: int, nnz_total: int, out=None, totals=None,
+                    cfg=None, stream=None):
+        torch = self._torch
+        G = int(tab.shape[0]) // 2 - 1
+        if out is None:
+            out = torch.empty(max(nv_total, 0), dtype=torch.int32, device=tab.device)
+        if totals is None:
+            totals = torch.empty(G, dtype=torch.int64, device=tab.device)
+        rc = lib().tc_batch_count_i32(self._h, G, _p(tab), nv_total, nnz_total, _p(off_cat),
+                                      _p(ind_cat) if nnz_total else None, _p(out) if nv_total else None,
+                                      _p(totals), _cfgp(cfg), _stream(stream))
+        self._check(rc, "tc_batch_count_i32")
+        return out, totals
+
+    # ------------------------------------------------------------ host convenience
+    def to_device(self, g: Graph):
+        torch = self._torch
+        dev = torch.device("cuda", self.device)
+        return (torch.from_numpy(np.ascontiguousarray(g.off)).to(dev),
+                torch.from_numpy(np.ascontiguousarray(g.ind)).to(dev))
+
+    def count_graph(self, g: Graph, cfg=None):
+        """H2D, count, D2H: ``(out int32[n], total int)`` like singleParamTestGPURun."""
+        off, ind = self.to_device(g)
+        out, total = self.count(off, ind, cfg=cfg)
+        self._torch.cuda.synchronize(self.device)
+        return out.cpu().numpy(), int(total.cpu().item())
+
+    def batch_to_device(self, b: Batch):
+        torch = self._torch
+        dev = torch.device("cuda", self.device)
+        return (torch.from_numpy(b.tab).to(dev), torch.from_numpy(b.off).to(dev), torch.from_numpy(b.ind).to(dev))
+
+    def count_batch(self, b: Batch, cfg=None):
+        tab, off, ind = self.batch_to_device(b)
+        out, totals = self.batch_count(tab, off, ind, b.total_vertices, b.total_arcs, cfg=cfg)
+        self._torch.cuda.synchronize(self.device)
+        return out.cpu().numpy(), totals.cpu().numpy()
+
+
+def kernel_call(numberBlocks: int, numberThreads: int, nv: int, d_off, d_ind, out, threads_per_block: int,
+                number_blocks: int, shifter: int) -> int:
+    """``kernelCall<int32_t>`` with the reference argument list (clusteringCount.cu:415-467).
+
+    Returns the C return code (0 on success, TC_EINVAL for a launch shape the
+    reference would not accept)."""
+    return int(lib().tc_kernelCall_i32(numberBlocks, numberThreads, nv, _p(d_off), _p(d_ind), _p(out),
+                                       threads_per_block, number_blocks, shifter))

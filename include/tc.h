@@ -1,0 +1,146 @@
+/*
+ * tc.h — C-ABI of the MI355X (gfx950) per-vertex triangle counter.
+ *
+ * Drop-in for the reference GPU path:
+ *   kernelCall<T>(blocks, bs, nv, d_off, d_ind, out, tpi, groups, shifter)
+ *       src/clusteringCount.cu:415-467, declared include/clusteringCount.cuh:99-104
+ *   count_all_trianglesGPU<T, bs>  (the __global__ it launches)
+ *       src/clusteringCount.cu:314-359
+ *   thrust::reduce(out) total in the caller
+ *       src/gpurun.cu:40-41
+ *
+ * Contract kept from the reference: CSR in (device int32 row offsets
+ * off[nv+1] and sorted neighbour ids ind[nnz]), per-vertex counts out
+ * (out[v] = sum over neighbours w of |N(v) ∩ N(w)| = 2·T(v), the same value
+ * the reference kernel and CPU path produce), every out[v] written, launch
+ * asynchronous on a stream.  Changes: the total is int64 (the reference
+ * sums in T and overflows at scale), errors are reported as return codes,
+ * and the input must be a simple undirected graph — rows strictly
+ * increasing, no self-loops, every arc (u,w) matched by (w,u) — which
+ * tc_validate_i32 checks on the device.  Per-vertex int32 is enough for every
+ * configuration measured (largest 2·T(v) at R-MAT scale 24 is ~2.0e8).
+ *
+ * Streams are passed as void* (a hipStream_t; NULL = the default stream).
+ * No torch or HIP types appear in the signatures.
+ */
+#ifndef TC_H
+#define TC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc_ctx tc_ctx;
+
+/* Launch configuration of the two-level Merge Path kernel.  Zero fields
+ * take the tuned default.
+ *   wg_size        threads per workgroup (256 or 512)
+ *   items          merge-path steps per lane per tile (8, 16 or 32)
+ *   ring           LDS ring length per adjacency list, elements (power of 2);
+ *                  a tile covers at most ring-2 steps of one long edge
+ *   small_max      edges with at most this much work (|N(u)|+|N(w)|) are
+ *                  packed several to a tile; longer ones stream through the
+ *                  LDS rings
+ *   wg_per_cu      level-1 split: the edge-work range is cut into
+ *                  wg_per_cu * CUs equal parts (one per workgroup)
+ *   algo           0 = two-level Merge Path (default), 1 = one lane per
+ *                  edge (simple, for cross-checks) */
+typedef struct tc_cfg {
+  int32_t wg_size;
+  int32_t items;
+  int32_t ring;
+  int32_t small_max;
+  int32_t wg_per_cu;
+  int32_t algo;
+} tc_cfg;
+
+enum {
+  TC_OK = 0,
+  TC_EINVAL = -1,    /* bad argument or unsupported launch configuration */
+  TC_ERANGE = -2,    /* sizes beyond the int32 path (nnz >= 2^31) */
+  TC_EHIP = -3,      /* a HIP runtime call failed (tc_last_error has the text) */
+  TC_ENOMEM = -4,    /* device scratch could not be allocated */
+  TC_EGRAPH = -5     /* tc_validate_i32: input is not a simple undirected CSR */
+};
+
+/* Context: owns device scratch (edge list, work prefix, scan partials), the
+ * tuned configuration table and timing events.  One context per host
+ * thread.  device < 0 uses the current device. */
+int tc_ctx_create(int device, tc_ctx** out);
+int tc_ctx_destroy(tc_ctx* ctx);
+const char* tc_last_error(const tc_ctx* ctx);
+/* Scratch bytes tc_count_i32 needs for a graph of this size (so a caller
+ * can reserve up front; tc_ctx_reserve allocates it). */
+size_t tc_scratch_bytes(int32_t nv, int64_t nnz);
+int tc_ctx_reserve(tc_ctx* ctx, size_t bytes);
+/* Default configuration used when cfg == NULL. */
+int tc_ctx_set_default_cfg(tc_ctx* ctx, const tc_cfg* cfg);
+int tc_ctx_get_default_cfg(const tc_ctx* ctx, tc_cfg* cfg);
+/* When enabled, each count records HIP events around its phases on the
+ * launch stream; tc_last_timing returns (after the stream is synchronised)
+ * ms[0] = edge-list prep, ms[1] = the Merge Path count kernel,
+ * ms[2] = finalisation, ms[3] = whole call. */
+int tc_ctx_set_timing(tc_ctx* ctx, int enable);
+int tc_last_timing(tc_ctx* ctx, float ms[4]);
+
+/* Per-vertex triangle counts of a simple undirected graph.
+ *   d_off[nv+1], d_ind[nnz]  device CSR (int32), rows sorted ascending
+ *   d_out[nv]                device int32, fully overwritten: 2·T(v)
+ *   d_total                  device int64 or NULL: sum of d_out (= 6·T)
+ * Replaces kernelCall<int32_t> + thrust::reduce (gpurun.cu:36-41). */
+int tc_count_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                 int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, void* stream);
+
+/* Shape-compatible stand-in for kernelCall<int32_t>
+ * (src/clusteringCount.cu:415-467): same argument list and meaning.
+ * numberThreads must be one of the reference block sizes
+ * {32,64,96,128,192,256,512,1024} and threads_per_block (threads per
+ * intersection) one of {1,2,4,8,16,32} with number_blocks == numberThreads /
+ * threads_per_block and shifter == log2(threads_per_block), as
+ * param.cu:225-247 requires; otherwise TC_EINVAL (the reference silently
+ * launches nothing).  The legacy launch shape is validated, then the tuned
+ * two-level kernel runs on the default stream of the current device with a
+ * library-owned context.  outPutTriangles is fully written. */
+int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv, const int32_t* d_off,
+                      const int32_t* d_ind, int32_t* outPutTriangles, int32_t threads_per_block,
+                      int32_t number_blocks, int32_t shifter);
+
+/* FP64 local clustering coefficient lcc[v] = out[v] / (d(v)·(d(v)-1)),
+ * 0 when d(v) < 2; one IEEE division of exactly converted integers. */
+int tc_lcc_f64(tc_ctx* ctx, int32_t nv, const int32_t* d_off, const int32_t* d_out, double* d_lcc,
+               void* stream);
+
+/* Device-side check of the simple-graph contract.  *h_status (host) gets 0
+ * when valid, else TC_EGRAPH; the call synchronises the stream. */
+int tc_validate_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                    int* h_status, void* stream);
+
+/* Batched count: G graphs packed as one concatenated CSR (layout of
+ * tch_batch in tc_host.h):
+ *   d_tab[2*(G+1)]  int64 {v_base, arc_base} per graph, entry G = totals
+ *   d_off_cat       int32, graph g's local offsets at d_off_cat[v_base+g ..]
+ *   d_ind_cat       int32 local ids, graph g's at d_ind_cat[arc_base ..]
+ *   d_out_cat       int32[total vertices], graph g's counts at [v_base ..]
+ *   d_totals        int64[G] or NULL: per-graph sum of counts
+ * h_nv_total / h_nnz_total are tab[2G] / tab[2G+1] (host copies, used to
+ * size scratch and launches without a device read).  One launch sequence
+ * for all graphs; results equal counting each graph alone. */
+int tc_batch_count_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t h_nv_total, int64_t h_nnz_total,
+                       const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals,
+                       const tc_cfg* cfg, void* stream);
+size_t tc_batch_scratch_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
+
+/* Global-count mode (README.md:6-10): only the int64 total 6·T, via a
+ * degree-ordered orientation so each triangle is intersected once. */
+int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                       int64_t* d_total, void* stream);
+
+/* Number of compute units of the context's device (for sizing). */
+int tc_device_cus(const tc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

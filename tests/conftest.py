@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    config.addinivalue_line("markers", "slow: larger graphs (seconds of CPU oracle time)")
+
+
+@pytest.fixture(scope="session")
+def counter():
+    from gputrianglecounting_amd.tc import TriangleCounter
+
+    c = TriangleCounter(0)
+    yield c
+    c.close()

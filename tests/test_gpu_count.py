@@ -1,0 +1,135 @@
+"""GPU parity: the HIP counter (through the C-ABI) against the CPU oracle.
+
+Bit-exact per-vertex int32 counts and int64 totals on closed-form and seeded
+random graphs, every compiled kernel variant, the one-lane-per-edge
+cross-check kernel, LCC in FP64, the kernelCall stand-in, validation, the
+global-count mode and batches."""
+import numpy as np
+import pytest
+
+import oracle
+from gputrianglecounting_amd import graph as G
+from gputrianglecounting_amd import tc as T
+from tests import graphs as GR
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(256, 16, 4096), (256, 8, 2048), (512, 8, 4096), (256, 32, 8192), (128, 32, 4096)]
+
+
+def _check(counter, g, cfg=None):
+    print(f"case {g.name} n={g.n} nnz={g.nnz} cfg={cfg}", flush=True)
+    out, tot = counter.count_graph(g, cfg=cfg)
+    ref, rtot = oracle.per_vertex(g.off, g.ind)
+    assert out.dtype == np.int32 and out.shape == (g.n,)
+    np.testing.assert_array_equal(out, ref)
+    assert tot == rtot == int(out.astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("g", GR.small_cases(), ids=lambda g: g.name)
+def test_small_closed_forms(counter, g):
+    _check(counter, g)
+
+
+@pytest.mark.parametrize("g", GR.random_cases(), ids=lambda g: g.name)
+def test_random_graphs(counter, g):
+    _check(counter, g)
+
+
+@pytest.mark.parametrize("v", VARIANTS, ids=lambda v: "x".join(map(str, v)))
+@pytest.mark.parametrize("small_max", [16, 300, 0])
+@pytest.mark.parametrize("wg_per_cu", [1, 16])
+def test_variants(counter, v, small_max, wg_per_cu):
+    g = G.rmat(13, 16, seed=21)
+    cfg = dict(wg_size=v[0], items=v[1], ring=v[2], small_max=small_max, wg_per_cu=wg_per_cu)
+    _check(counter, g, cfg)
+
+
+def test_simple_kernel_agrees(counter):
+    g = G.rmat(12, 16, seed=4)
+    _check(counter, g, dict(algo=1))
+
+
+def test_repeat_deterministic(counter):
+    g = G.rmat(14, 16, seed=8)
+    a, ta = counter.count_graph(g)
+    b, tb = counter.count_graph(g)
+    np.testing.assert_array_equal(a, b)
+    assert ta == tb
+
+
+def test_lcc_bit_exact(counter):
+    import torch
+
+    g = G.rmat(12, 16, seed=2)
+    off, ind = counter.to_device(g)
+    out, _ = counter.count(off, ind)
+    lcc = counter.lcc(off, out)
+    torch.cuda.synchronize()
+    ref_out, _ = oracle.per_vertex(g.off, g.ind)
+    ref = oracle.lcc(g.off, ref_out)
+    got = lcc.cpu().numpy()
+    assert got.view(np.int64).tolist() == ref.view(np.int64).tolist()
+
+
+def test_kernel_call_shim(counter):
+    import torch
+
+    g = G.erdos_renyi(2000, 20000, seed=5)
+    off, ind = counter.to_device(g)
+    out = torch.full((g.n,), -7, dtype=torch.int32, device=off.device)
+    assert T.kernel_call(64, 256, g.n, off, ind, out, 4, 64, 2) == 0
+    torch.cuda.synchronize()
+    ref, _ = oracle.per_vertex(g.off, g.ind)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    # launch shapes the reference rejects (param.cu:225-247) are errors here
+    assert T.kernel_call(64, 100, g.n, off, ind, out, 4, 25, 2) == T.TC_EINVAL
+    assert T.kernel_call(64, 256, g.n, off, ind, out, 3, 85, 1) == T.TC_EINVAL
+    assert T.kernel_call(64, 256, g.n, off, ind, out, 4, 32, 2) == T.TC_EINVAL
+
+
+def test_validate(counter):
+    import torch
+
+    g = G.rmat(10, 8, seed=3)
+    off, ind = counter.to_device(g)
+    assert counter.validate(off, ind) == 0
+    bad = g.ind.copy()
+    r = int(np.argmax(np.diff(g.off) >= 3))
+    k = int(g.off[r])
+    bad[k], bad[k + 1] = bad[k + 1], bad[k]  # unsorted row
+    assert counter.validate(off, torch.from_numpy(bad).to(off.device)) == T.TC_EGRAPH
+    sl = g.ind.copy()
+    sl[k] = r  # self-loop (and a missing reverse)
+    assert counter.validate(off, torch.from_numpy(sl).to(off.device)) == T.TC_EGRAPH
+    assert "self-loop" in counter.last_error()
+
+
+def test_global_count_mode(counter):
+    for g in [G.rmat(14, 16, seed=6), GR.complete(40), G.geometric(30000, 6.0, seed=1), GR.empty(3)]:
+        off, ind = counter.to_device(g)
+        tot = counter.count_total(off, ind)
+        _, rtot = oracle.per_vertex(g.off, g.ind)
+        assert int(tot.cpu().item()) == rtot, g.name
+
+
+def test_batch_equals_alone(counter):
+    b = G.gen_batch(G=96, n_lo=16, n_hi=3000, seed=3)
+    out, totals = counter.count_batch(b)
+    for g in range(b.count):
+        gr = b.graph(g)
+        ref, rtot = oracle.per_vertex(gr.off, gr.ind)
+        vb = b.v_base(g)
+        np.testing.assert_array_equal(out[vb: vb + gr.n], ref, err_msg=f"graph {g}")
+        assert totals[g] == rtot
+
+
+def test_errors(counter):
+    import torch
+
+    off = torch.zeros(4, dtype=torch.int32, device="cuda")
+    ind = torch.zeros(0, dtype=torch.int32, device="cuda")
+    with pytest.raises(T.TCError):
+        counter.count(off, ind, cfg=dict(wg_size=100))
+    with pytest.raises(T.TCError):
+        counter.count(off, ind, cfg=dict(wg_per_cu=1000))
