@@ -1,0 +1,211 @@
+"""Benchmark: per-vertex triangle counting on one MI355X.
+
+Default workload: R-MAT scale 24, edge factor 16 (a=.57 b=.19 c=.19),
+seed 1, simple symmetric int32 CSR (~520M arcs), counted by the two-level
+Merge Path kernel.  A step is one full tc_count_i32 call over the resident
+graph: edge-list prep, the count kernel and the int64 total.  The metric is
+undirected edges / step time (edge-intersections/s); the roofline entry is
+for the count kernel, timed with HIP events on its launch stream.
+
+The CPU baseline runs the reference's own intersection (oracle/_ref, or the
+C restatement when that is absent) on a bounded uniform sample of arcs of
+the same graph on one host core; its full-graph time is extrapolated by arc
+count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "edge-intersections/s + HBM GB/s (R-MAT s24); batched vs per-graph graphs/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def make_graph(workload: str):
+    from gputrianglecounting_amd import graph as G
+
+    if workload == "rmat24":
+        return G.rmat(24, 16, seed=1), dict(workload="rmat_s24_ef16", scale=24, edge_factor=16, a=0.57, b=0.19,
+                                            c=0.19, seed=1)
+    if workload == "rmat20":
+        return G.rmat(20, 16, seed=1), dict(workload="rmat_s20_ef16", scale=20, edge_factor=16, a=0.57, b=0.19,
+                                            c=0.19, seed=1)
+    if workload == "rgg":
+        return G.geometric(4194304, 6.0, seed=1), dict(workload="rgg_n4194304_deg6", seed=1)
+    if workload == "er":
+        return G.erdos_renyi(16384, 131072, seed=1), dict(workload="er_n16384_m131072", seed=1)
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def algorithmic_bytes(g, W: int) -> dict:
+    """Bytes the count moves by definition: every intersected edge streams both
+    sorted lists once (4 B per element, W elements), plus reading the CSR once
+    and writing int32 counts and the int64 total.  ``b_alg_all`` is the same
+    with every undirected edge intersected (4 * sum d^2)."""
+    d = g.degrees().astype(np.float64)
+    fixed = 4 * (g.n + 1) + 4 * g.nnz + 4 * g.n + 8
+    return dict(b_kernel=4 * W + fixed, b_alg_all=4 * float((d * d).sum()) + fixed)
+
+
+def cpu_baseline(g, target_s: float) -> dict:
+    import oracle
+
+    use_ref = oracle.ref_lib() is not None
+    rng = np.random.default_rng(12345)
+
+    def run(k: int):
+        arcs = np.sort(rng.choice(g.nnz, size=k, replace=False)).astype(np.int64)
+        t0 = time.perf_counter()
+        s = oracle.count_arc_sample(g.off, g.ind, arcs, use_ref=use_ref)
+        return time.perf_counter() - t0, s
+
+    k = 20000
+    t, _ = run(k)
+    k2 = int(min(g.nnz, max(k, k * target_s / max(t, 1e-3))))
+    t, _ = run(k2)
+    full_s = t * g.nnz / k2
+    edges = g.nnz // 2
+    cpu = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": edges / full_s, "unit": "edge-intersections/s", "cores": 1,
+            "kind": "reference" if use_ref else "port",
+            "sample": (f"{k2} arcs drawn uniformly (seed 12345) from the same graph, visited in CSR order by "
+                       f"{'the reference count_intersectionsReg (clusteringCount.cpp:2-48)' if use_ref else 'the C restatement'}"
+                       f" on 1 core ({cpu}); {t:.1f} s, full-graph time extrapolated by arc count: {full_s:.0f} s")}
+
+
+def traffic_from_profile(workload: str):
+    path = os.path.join(ROOT, "profiles", f"count_mp_{workload}_hbm.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=16,ring=4096")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the count runs on a single MI355X (no multi-GPU path is built); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from gputrianglecounting_amd.tc import TriangleCounter
+
+    log(f"generating {args.workload}")
+    t0 = time.time()
+    g, config = make_graph(args.workload)
+    log(f"graph n={g.n} arcs={g.nnz} in {time.time() - t0:.1f}s")
+
+    cfg = None
+    if args.cfg:
+        cfg = {k: int(v) for k, v in (kv.split("=") for kv in args.cfg.split(","))}
+
+    tc = TriangleCounter(0)
+    dev = torch.device("cuda", 0)
+    off = torch.from_numpy(g.off).to(dev)
+    ind = torch.from_numpy(g.ind).to(dev)
+    out = torch.empty(g.n, dtype=torch.int32, device=dev)
+    total = torch.empty(1, dtype=torch.int64, device=dev)
+    tc.set_timing(True)
+    torch.cuda.synchronize()
+    log("uploaded; warmup")
+
+    for _ in range(args.warmup):
+        tc.count(off, ind, out, total, cfg=cfg)
+        torch.cuda.synchronize()
+        log(f"warmup step {tc.last_timing()}")
+    E, W = tc.last_edge_work()
+
+    # full-size checks (untimed): invariants and an independent total from
+    # the degree-ordered global-count mode
+    h_out = out.cpu().numpy().astype(np.int64)
+    d = g.degrees()
+    tot = int(total.cpu().item())
+    inv_ok = bool((h_out % 2 == 0).all() and (h_out <= d * (d - 1)).all() and (h_out >= 0).all()
+                  and tot == int(h_out.sum()) and tot % 6 == 0)
+    tot_global = int(tc.count_total(off, ind).cpu().item())
+    log(f"checks: invariants={inv_ok} total={tot} global-mode total={tot_global}")
+
+    torch.cuda.synchronize()
+    kern = []
+    prep = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        tc.count(off, ind, out, total, cfg=cfg)
+        tm = tc.last_timing()  # waits for this step's events
+        kern.append(tm.count_ms)
+        prep.append(tm.prep_ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, count kernel {np.mean(kern):.1f} ms")
+
+    edges = g.nnz // 2
+    value = edges / (ms_per_step / 1e3)
+    ab = algorithmic_bytes(g, W)
+    t_k = float(np.mean(kern)) / 1e3
+    achieved = ab["b_kernel"] / t_k / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(config["workload"])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(g, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3g} {cpu['unit']}")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "edge-intersections/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: seeded R-MAT/ER/RGG from the package's counter-based generators",
+        "config": dict(config, n=g.n, arcs=g.nnz, undirected_edges=edges, intersected_edges=E, merge_work=W,
+                       kernel_cfg=tc.default_cfg() if cfg is None else cfg),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "count_kernel_ms": round(float(np.mean(kern)), 3),
+        "prep_ms": round(float(np.mean(prep)), 3),
+        "b_kernel_bytes": ab["b_kernel"],
+        "b_alg_all_edges_bytes": ab["b_alg_all"],
+        "hbm_gbs_all_edges_formula": round(ab["b_alg_all"] / t_k / 1e9, 1),
+        "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
+                   "totals_agree": tot == tot_global, "triangles": tot // 6},
+    }
+    print(json.dumps(line), flush=True)
+    tc.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -22,6 +22,7 @@ struct tc_ctx {
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool ev_recorded = false;
+  const int64_t* last_dscal = nullptr;  // E, W of the last count (device)
 };
 
 namespace {
@@ -114,6 +115,7 @@ int ensure(tc_ctx* c, size_t bytes) {
     hipError_t e = hipFree(c->scratch);
     c->scratch = nullptr;
     c->cap = 0;
+    c->last_dscal = nullptr;
     if (e != hipSuccess) return hip_fail(c, e, "hipFree");
   }
   size_t want = bytes + bytes / 8;
@@ -274,6 +276,15 @@ int tc_last_timing(tc_ctx* c, float ms[4]) {
 
 int tc_device_cus(const tc_ctx* c) { return c ? c->cus : 0; }
 
+int tc_last_edge_work(tc_ctx* c, int64_t out[2]) {
+  if (!c || !out) return TC_EINVAL;
+  if (!c->scratch || !c->last_dscal) return set_err(c, TC_EINVAL, "no count has run on this context");
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, c->last_dscal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? TC_OK : hip_fail(c, e, "tc_last_edge_work");
+}
+
 int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind, int32_t* d_out,
                  int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
   if (!c) return TC_EINVAL;
@@ -299,6 +310,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   record(c, 1, s);
   rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
   if (rc) return rc;
+  c->last_dscal = p.dscal;
   record(c, 2, s);
   record(c, 3, s);
   c->ev_recorded = c->timing;
@@ -332,6 +344,7 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   record(c, 1, s);
   rc = run_count(c, q, cfg, vi, p.oind, nullptr, d_total, 6, s);
   if (rc) return rc;
+  c->last_dscal = p.dscal;
   record(c, 2, s);
   record(c, 3, s);
   c->ev_recorded = c->timing;
@@ -365,6 +378,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
   record(c, 1, s);
   rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
   if (rc) return rc;
+  c->last_dscal = p.dscal;
   record(c, 2, s);
   if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
     return hip_fail(c, e, "totals");

@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
         "tc_batch_count_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_count_total_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
         "tc_device_cus": (C.c_int, [vp]),
+        "tc_last_edge_work": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -89,7 +90,7 @@ def lib() -> C.CDLL:
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
            "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
-           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus"]
+           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus", "tc_last_edge_work"]
 
 
 def _torch():
@@ -177,6 +178,12 @@ class TriangleCounter:
         ms = (C.c_float * 4)()
         self._check(lib().tc_last_timing(self._h, ms), "timing")
         return Timing(*[float(x) for x in ms])
+
+    def last_edge_work(self):
+        """(E, W) of the last count: edges intersected and their total Merge Path length."""
+        buf = (C.c_int64 * 2)()
+        self._check(lib().tc_last_edge_work(self._h, buf), "edge work")
+        return int(buf[0]), int(buf[1])
 
     def reserve(self, nbytes: int) -> None:
         self._check(lib().tc_ctx_reserve(self._h, nbytes), "reserve")

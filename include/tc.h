@@ -137,6 +137,11 @@ size_t tc_batch_scratch_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
 int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
                        int64_t* d_total, void* stream);
 
+/* Edge count E and total Merge Path work W = sum over intersected edges of
+ * |N(u)| + |N(w)| of the last count on this context (synchronises the
+ * context's device).  out[0] = E, out[1] = W. */
+int tc_last_edge_work(tc_ctx* ctx, int64_t out[2]);
+
 /* Number of compute units of the context's device (for sizing). */
 int tc_device_cus(const tc_ctx* ctx);
 

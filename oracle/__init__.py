@@ -44,6 +44,8 @@ def lib():
         L.oracle_per_vertex.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I64P]
         L.oracle_per_vertex_omp.restype = C.c_int
         L.oracle_per_vertex_omp.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I64P, C.c_int32]
+        L.oracle_count_arc_sample.restype = C.c_int64
+        L.oracle_count_arc_sample.argtypes = [_I32P, _I32P, _I32P, _I64P, C.c_int64]
         L.oracle_lcc.restype = None
         L.oracle_lcc.argtypes = [C.c_int32, _I32P, _I32P, C.POINTER(C.c_double)]
         _o = L
@@ -68,6 +70,8 @@ def ref_lib():
         L.tcref_read_info.argtypes = [C.c_char_p, _I32P, _I32P, _I32P, C.c_int64, _I32P, C.c_int64]
         L.tcref_read_metis.restype = C.c_int
         L.tcref_read_metis.argtypes = [C.c_char_p, _I32P, _I64P, _I32P, C.c_int64, _I32P, C.c_int64]
+        L.tcref_count_arc_sample.restype = C.c_int64
+        L.tcref_count_arc_sample.argtypes = [_I32P, _I32P, _I32P, _I64P, C.c_int64]
         L.tcref_write_info.restype = None
         L.tcref_write_info.argtypes = [C.c_char_p, C.c_int32, _I32P, _I32P]
         _r = L
@@ -146,3 +150,16 @@ def row_sums(off, tri):
     off = np.asarray(off, np.int64)
     cs = np.concatenate([[0], np.cumsum(tri, dtype=np.int64)])
     return cs[off[1:]] - cs[off[:-1]]
+
+
+def count_arc_sample(off, ind, arcs, use_ref=True):
+    """Sum of per-arc intersections over the arc indices ``arcs`` (visited in
+    the given order) with the reference code (``use_ref``) or the port."""
+    off, ind = _c(off, np.int32), _c(ind, np.int32)
+    arcs = _c(arcs, np.int64)
+    src = _c(np.searchsorted(off, arcs, side="right") - 1, np.int32)
+    L = ref_lib() if use_ref else lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref not built")
+    fn = L.tcref_count_arc_sample if use_ref else L.oracle_count_arc_sample
+    return int(fn(_p32(off), _p32(ind), _p32(src), _p64(arcs), arcs.shape[0]))

@@ -30,6 +30,20 @@ void tcref_count_vertex_list(int32_t count, const int32_t* list, const int32_t* 
                              int64_t* triNE, int64_t* total) {
   count_all_trianglesCPU<int32_t>(count, list, off, ind, triNE, total);
 }
+// Bounded CPU baseline: the reference intersection (count_intersectionsReg,
+// clusteringCount.cpp:2-48) on a caller-chosen list of arcs (k, src[k]) in
+// the order given, exactly as count_all_trianglesCPU calls it per arc
+// (clusteringCount.cpp:57-68).  Returns the sum of the per-arc counts.
+int64_t tcref_count_arc_sample(const int32_t* off, const int32_t* ind, const int32_t* src, const int64_t* arc,
+                               int64_t count) {
+  int64_t sum = 0;
+  for (int64_t s = 0; s < count; ++s) {
+    const int32_t u = src[s];
+    const int32_t w = ind[arc[s]];
+    sum += count_intersectionsReg<int32_t>(u, off[u + 1] - off[u], ind + off[u], w, off[w + 1] - off[w], ind + off[w]);
+  }
+  return sum;
+}
 // readGraph<int32_t>: returns nv and nnz; copies into caller buffers of the
 // given capacity (call once with zero capacity to size them).
 int tcref_read_info(const char* path_no_ext, int32_t* nv, int32_t* nnz, int32_t* off, int64_t off_cap,

@@ -20,6 +20,8 @@
  *       per source vertex (src/clusteringCount.cu:331-357), which is 2*T(v)
  *       on a simple graph.
  *   oracle_per_vertex_omp        same numbers, rows split over OpenMP threads.
+ *   oracle_count_arc_sample      the per-arc intersection of clusteringCount.cpp:57-68
+ *       on a caller-chosen arc list (bounded CPU baseline).
  *   oracle_lcc                   local clustering coefficient
  *       out[v] / (d(v) * (d(v) - 1)) in IEEE double, 0 for d < 2 (no
  *       reference counterpart; derived value).
@@ -100,6 +102,18 @@ void oracle_count_vertex_list(int32_t count, const int32_t* vertexList, const in
     }
   }
   *total = sum;
+}
+
+/* Per-arc intersections for a caller-chosen arc list (arc index arc[s] whose
+ * source row is src[s]), in the order given; the bounded CPU baseline. */
+int64_t oracle_count_arc_sample(const int32_t* off, const int32_t* ind, const int32_t* src, const int64_t* arc,
+                                int64_t count) {
+  int64_t sum = 0;
+  for (int64_t s = 0; s < count; ++s) {
+    int32_t u = src[s], w = ind[arc[s]];
+    sum += oracle_count_intersections(u, off[u + 1] - off[u], ind + off[u], w, off[w + 1] - off[w], ind + off[w]);
+  }
+  return sum;
 }
 
 /* Per-vertex row sums of the per-arc counts.  Returns 0, or -1 when a row

@@ -1,0 +1,257 @@
+"""Pins for the CPU oracle (oracle/tc_oracle.c) — CPU only.
+
+The oracle is the checker of every GPU parity test, so it is itself pinned
+against independent answers:
+  * the compiled reference CPU code (oracle/_ref) when it is built, and the
+    golden vectors that code wrote (tests/golden, always available);
+  * closed forms for K_n, cycles, wheels, friendship graphs, bipartite and
+    triangle-free graphs;
+  * diag(A^3) by a dense integer matrix product (n <= 200) and by a scipy
+    sparse product (A^2 ∘ A row sums) on the seeded ER and R-MAT graphs;
+  * networkx.triangles and networkx.clustering;
+  * for non-simple rows (self-loops), a set formula of the reference's
+    self-id skip: |(N(v)\\{v}) ∩ (N(w)\\{w})| per arc.
+"""
+import glob
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from gputrianglecounting_amd import graph as G
+from tests import graphs as GR
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _adj_dense(g):
+    A = np.zeros((g.n, g.n), np.int64)
+    for v in range(g.n):
+        A[v, g.ind[g.off[v]:g.off[v + 1]]] = 1
+    return A
+
+
+def _sparse(g):
+    import scipy.sparse as sp
+
+    return sp.csr_matrix((np.ones(g.nnz, np.int64), g.ind.astype(np.int64), g.off.astype(np.int64)),
+                         shape=(g.n, g.n))
+
+
+# ---------------------------------------------------------------- golden
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "arcs_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_arcs_golden(path):
+    d = np.load(path)
+    tri, tot = oracle.count_arcs(d["off"], d["ind"])
+    np.testing.assert_array_equal(tri, d["tri"])
+    assert tot == int(d["total"])
+
+
+def _gen(name):
+    return {
+        "er_n16384_m131072_s1": lambda: G.erdos_renyi(16384, 131072, seed=1),
+        "rmat_s12_ef16_s1": lambda: G.rmat(12, 16, seed=1),
+        "rgg_n20000_d6_s1": lambda: G.geometric(20000, 6.0, seed=1),
+    }[name]()
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "vertex_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_vertex_golden(path):
+    d = np.load(path)
+    name = os.path.basename(path)[len("vertex_"):-4]
+    g = _gen(name)
+    h = hashlib.sha256()
+    h.update(g.off.tobytes())
+    h.update(g.ind.tobytes())
+    assert h.hexdigest() == str(d["sha256"]), "generator output drifted from the fixture"
+    out, tot = oracle.per_vertex(g.off, g.ind)
+    np.testing.assert_array_equal(out, d["out"])
+    assert tot == int(d["total"])
+
+
+def test_reference_build_agrees_on_random_graphs():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent); golden vectors cover this")
+    rng = np.random.default_rng(7)
+    for t in range(30):
+        n = int(rng.integers(2, 80))
+        rows = [sorted(rng.integers(0, n, size=int(rng.integers(0, 12))).tolist()) for _ in range(n)]
+        off = np.zeros(n + 1, np.int32)
+        for i, r in enumerate(rows):
+            off[i + 1] = off[i] + len(r)
+        ind = np.array([x for r in rows for x in r], np.int32)
+        a, ta = oracle.count_arcs(off, ind)
+        b, tb = oracle.ref_count_arcs(off, ind)
+        np.testing.assert_array_equal(a, b)
+        assert ta == tb
+    for g in GR.random_cases()[:5]:
+        a, ta = oracle.count_arcs(g.off, g.ind)
+        b, tb = oracle.ref_count_arcs(g.off, g.ind)
+        np.testing.assert_array_equal(a, b)
+        assert ta == tb
+
+
+def test_reference_vertex_list_agrees():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built")
+    g = G.rmat(11, 16, seed=2)
+    vl = np.array([5, 0, 77, 1500, 3], np.int32)
+    tri, tot = oracle.ref_count_vertex_list(vl, g.off, g.ind)
+    full, _ = oracle.count_arcs(g.off, g.ind)
+    expect = np.concatenate([full[g.off[v]:g.off[v + 1]] for v in vl])
+    np.testing.assert_array_equal(tri, expect)
+    assert tot == int(expect.sum())
+
+
+# ---------------------------------------------------------------- closed forms
+@pytest.mark.parametrize("n", [3, 4, 5, 12, 40])
+def test_complete(n):
+    out, tot = oracle.per_vertex(*_csr(GR.complete(n)))
+    assert (out == (n - 1) * (n - 2)).all()
+    assert tot == n * (n - 1) * (n - 2)
+
+
+def _csr(g):
+    return g.off, g.ind
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 17])
+def test_cycle(k):
+    out, tot = oracle.per_vertex(*_csr(GR.cycle(k)))
+    assert (out == (2 if k == 3 else 0)).all() and tot == (6 if k == 3 else 0)
+
+
+@pytest.mark.parametrize("k", [4, 5, 9, 30])
+def test_wheel(k):
+    out, _ = oracle.per_vertex(*_csr(GR.wheel(k)))
+    assert out[0] == 2 * k and (out[1:] == 4).all()
+
+
+@pytest.mark.parametrize("k", [1, 2, 7])
+def test_friendship(k):
+    out, tot = oracle.per_vertex(*_csr(GR.friendship(k)))
+    assert out[0] == 2 * k and (out[1:] == 2).all() and tot == 6 * k
+
+
+@pytest.mark.parametrize("g", [GR.complete_bipartite(3, 5), GR.petersen(), GR.grid(6, 7), GR.star(10), GR.empty(4)],
+                         ids=lambda g: g.name)
+def test_triangle_free(g):
+    out, tot = oracle.per_vertex(*_csr(g))
+    assert (out == 0).all() and tot == 0
+
+
+def test_two_hubs():
+    m = 50
+    out, _ = oracle.per_vertex(*_csr(GR.two_hubs(m)))
+    assert out[0] == 2 * m and out[1] == 2 * m and (out[2:] == 2).all()
+
+
+# ---------------------------------------------------------------- matrix products
+def test_dense_a_cubed():
+    rng = np.random.default_rng(3)
+    for t in range(12):
+        n = int(rng.integers(5, 200))
+        m = int(rng.integers(n, 6 * n))
+        g = G.erdos_renyi(n, m, seed=100 + t)
+        A = _adj_dense(g)
+        diag = np.einsum("ij,ji->i", A @ A, A)
+        out, tot = oracle.per_vertex(g.off, g.ind)
+        np.testing.assert_array_equal(out, diag)
+        assert tot == int(np.trace(A @ A @ A))
+
+
+@pytest.mark.parametrize("g", [G.erdos_renyi(16384, 131072, seed=1), G.rmat(14, 16, seed=11),
+                               G.geometric(50000, 6.0, seed=4)], ids=lambda g: g.name)
+def test_sparse_a_cubed(g):
+    A = _sparse(g)
+    per = np.asarray((A @ A).multiply(A).sum(axis=1)).ravel()
+    out, tot = oracle.per_vertex(g.off, g.ind)
+    np.testing.assert_array_equal(out, per)
+    assert tot == int(per.sum())
+
+
+def test_networkx_triangles_and_clustering():
+    nx = pytest.importorskip("networkx")
+    for g in [G.rmat(10, 16, seed=5), G.erdos_renyi(400, 3000, seed=2), GR.wheel(9)]:
+        H = nx.Graph()
+        H.add_nodes_from(range(g.n))
+        for v in range(g.n):
+            for w in g.ind[g.off[v]:g.off[v + 1]]:
+                H.add_edge(v, int(w))
+        tri = nx.triangles(H)
+        out, _ = oracle.per_vertex(g.off, g.ind)
+        np.testing.assert_array_equal(out, np.array([2 * tri[v] for v in range(g.n)]))
+        cl = nx.clustering(H)
+        lcc = oracle.lcc(g.off, out)
+        assert lcc.tolist() == [float(cl[v]) for v in range(g.n)]
+
+
+# ---------------------------------------------------------------- non-simple rows
+def _set_formula(off, ind):
+    """Per arc (v, w): |(N(v) minus {v}) ∩ (N(w) minus {w})| for rows without duplicates."""
+    res = []
+    n = off.shape[0] - 1
+    for v in range(n):
+        Nv = set(ind[off[v]:off[v + 1]].tolist()) - {v}
+        for w in ind[off[v]:off[v + 1]]:
+            Nw = set(ind[off[w]:off[w + 1]].tolist()) - {int(w)}
+            res.append(len(Nv & Nw))
+    return np.array(res, np.int64)
+
+
+def test_self_loop_skip_matches_set_formula():
+    rng = np.random.default_rng(11)
+    for t in range(40):
+        n = int(rng.integers(2, 40))
+        rows = []
+        for v in range(n):
+            r = set(rng.integers(0, n, size=int(rng.integers(0, 8))).tolist())
+            if rng.random() < 0.3:
+                r.add(v)  # self-loop
+            rows.append(sorted(r))
+        off = np.zeros(n + 1, np.int32)
+        for i, r in enumerate(rows):
+            off[i + 1] = off[i] + len(r)
+        ind = np.array([x for r in rows for x in r], np.int32)
+        tri, _ = oracle.count_arcs(off, ind)
+        np.testing.assert_array_equal(tri, _set_formula(off, ind))
+
+
+def test_intersections_edge_cases():
+    a = np.array([1, 3, 5, 7], np.int32)
+    assert oracle.intersections(0, a, 9, np.array([3, 7, 9], np.int32)) == 2
+    assert oracle.intersections(0, a, 9, np.zeros(0, np.int32)) == 0
+    assert oracle.intersections(0, a, 9, np.array([8, 9, 10], np.int32)) == 0  # disjoint ranges
+    assert oracle.intersections(3, a, 9, np.array([3, 5], np.int32)) == 1  # self id skipped in a
+
+
+def test_omp_variant_bit_identical():
+    for g in [G.rmat(13, 16, seed=3), G.geometric(30000, 6.0, seed=9)]:
+        a, ta = oracle.per_vertex(g.off, g.ind)
+        b, tb = oracle.per_vertex(g.off, g.ind, threads=4)
+        np.testing.assert_array_equal(a, b)
+        assert ta == tb
+
+
+def test_arc_sample_equals_full_count():
+    g = G.rmat(11, 16, seed=4)
+    _, tot = oracle.count_arcs(g.off, g.ind)
+    assert oracle.count_arc_sample(g.off, g.ind, np.arange(g.nnz), use_ref=False) == tot
+    if oracle.ref_lib() is not None:
+        assert oracle.count_arc_sample(g.off, g.ind, np.arange(g.nnz), use_ref=True) == tot
+
+
+def test_planted_mistake_is_caught():
+    """A checker that drops the self-id skip would disagree with the golden
+    vectors of the self-loop fixture: make sure the fixture can tell."""
+    d = np.load(os.path.join(GOLD, "arcs_selfloops.npz"))
+    off, ind = d["off"], d["ind"]
+    naive = []
+    for v in range(off.shape[0] - 1):
+        for w in ind[off[v]:off[v + 1]]:
+            naive.append(len(set(ind[off[v]:off[v + 1]].tolist()) & set(ind[off[w]:off[w + 1]].tolist())))
+    assert not np.array_equal(np.array(naive), d["tri"])
