@@ -94,6 +94,63 @@ def cpu_baseline(g, target_s: float) -> dict:
                        f" on 1 core ({cpu}); {t:.1f} s, full-graph time extrapolated by arc count: {full_s:.0f} s")}
 
 
+def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
+    """The mixed batch: G graphs (n log-uniform in [64, 65536]; ER mean degree
+    8, R-MAT a=.57 b=.19 c=.19 ef 8-32, RGG mean degree 6) packed into one
+    CSR, counted by one captured HIP graph per step, against one
+    tc_count_i32 launch sequence per graph on the same GPU."""
+    import torch
+
+    from gputrianglecounting_amd import autotune as AT
+    from gputrianglecounting_amd import graph as G
+
+    t0 = time.time()
+    b = G.gen_batch(G=G_, n_lo=64, n_hi=65536, ef_lo=8, ef_hi=32, seed=7)
+    log(f"batch of {b.count} graphs, {b.total_vertices} vertices, {b.total_arcs} arcs in {time.time() - t0:.1f}s")
+    saved = tc.default_cfg()
+    tuned = AT.apply(tc, f"batch{G_}")
+    cfg = tc.default_cfg()
+    tab, off, ind = tc.batch_to_device(b)
+    out = torch.empty(b.total_vertices, dtype=torch.int32, device="cuda")
+    totals = torch.empty(b.count, dtype=torch.int64, device="cuda")
+    graph = tc.capture_batch(tab, off, ind, b.total_vertices, b.total_arcs, out, totals)
+    for _ in range(warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        graph.launch()
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t1) / steps
+    out_b, tot_b = out.clone(), totals.clone()
+    # one launch sequence per graph
+    out2 = torch.empty_like(out)
+    tot2 = torch.empty_like(totals)
+    tc.batch_count_pergraph(b.tab, off, ind, out2, tot2)  # warm
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(max(1, steps // 2)):
+        tc.batch_count_pergraph(b.tab, off, ind, out2, tot2)
+    torch.cuda.synchronize()
+    per_s = (time.perf_counter() - t2) / max(1, steps // 2)
+    same = bool(torch.equal(out_b, out2) and torch.equal(tot_b, tot2))
+    h = out_b.cpu().numpy().astype(np.int64)
+    dv = np.diff(b.off.astype(np.int64))
+    # per-vertex degrees: graph g's offsets sit at off[v_base[g] + g ...], so
+    # the difference at v_base[g+1] + g straddles two graphs and is dropped
+    keep = np.ones(dv.shape[0], bool)
+    keep[b.tab[2:2 * b.count:2] + np.arange(b.count - 1)] = False
+    deg = dv[keep]
+    inv = bool((h % 2 == 0).all() and (h <= deg * (deg - 1)).all())
+    graph.close()
+    tc.set_default_cfg(**saved)
+    return {"graphs": b.count, "vertices": b.total_vertices, "arcs": b.total_arcs,
+            "batched_graphs_per_s": b.count / batched_s, "batched_ms": batched_s * 1e3,
+            "per_graph_graphs_per_s": b.count / per_s, "per_graph_ms": per_s * 1e3,
+            "speedup": per_s / batched_s, "identical_to_per_graph": same, "invariants": inv,
+            "kernel_cfg": cfg, "autotuned": tuned, "captured": "hipGraph"}
+
+
 def traffic_from_profile(workload: str):
     path = os.path.join(ROOT, "profiles", f"count_mp_{workload}_hbm.json")
     if not os.path.exists(path):
@@ -111,7 +168,8 @@ def main() -> int:
     ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=16,ring=4096")
+    ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=8,ring=4096")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-graph comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the count runs on a single MI355X (no multi-GPU path is built); "
@@ -185,6 +243,16 @@ def main() -> int:
         cpu = cpu_baseline(g, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.3g} {cpu['unit']}")
 
+    batch = None
+    if not args.no_batch and args.workload == "rmat24":
+        log("batched vs per-graph")
+        try:
+            batch = run_batch(tc, steps=max(2, args.steps), warmup=1)
+            log(f"batched {batch['batched_graphs_per_s']:.0f} graphs/s, per-graph {batch['per_graph_graphs_per_s']:.0f}"
+                f" graphs/s, speedup {batch['speedup']:.2f}x")
+        except Exception as exc:  # the headline line must still print
+            batch = {"error": repr(exc)}
+
     line = {
         "metric": METRIC, "value": value, "unit": "edge-intersections/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -199,6 +267,7 @@ def main() -> int:
         "b_kernel_bytes": ab["b_kernel"],
         "b_alg_all_edges_bytes": ab["b_alg_all"],
         "hbm_gbs_all_edges_formula": round(ab["b_alg_all"] / t_k / 1e9, 1),
+        "batched": batch,
         "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
                    "totals_agree": tot == tot_global, "triangles": tot // 6},
     }

@@ -94,7 +94,28 @@ void usage() {
 
 }  // namespace
 
+// tc_main --copy <MiB>: two plain device copies of that size (the second
+// is the one to read from a counter profile), for byte-counter calibration.
+int copy_mode(long long mib) {
+  tc_ctx* ctx = nullptr;
+  if (mib <= 0 || tc_ctx_create(0, &ctx) != TC_OK) return 2;
+  const size_t n = (size_t)mib * 1024 * 1024 / 4;
+  int32_t *a = nullptr, *b = nullptr;
+  HIPCHECK(hipMalloc(&a, 4 * n));
+  HIPCHECK(hipMalloc(&b, 4 * n));
+  HIPCHECK(hipMemset(a, 1, 4 * n));
+  for (int r = 0; r < 2; ++r)
+    if (tc_copy_i32(ctx, b, a, (int64_t)n, nullptr) != TC_OK) return 1;
+  HIPCHECK(hipDeviceSynchronize());
+  std::printf("copied %lld MiB twice\n", mib);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  tc_ctx_destroy(ctx);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 3 && std::strcmp(argv[1], "--copy") == 0) return copy_mode(std::atoll(argv[2]));
   if (argc < 4) {
     usage();
     return 2;
@@ -189,7 +210,7 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "cannot write %s\n", fname.c_str());
       return 1;
     }
-    static const int kVar[][3] = {{256, 16, 4096}, {256, 8, 2048}, {512, 8, 4096}, {256, 32, 8192}, {128, 32, 4096}};
+    static const int kVar[][3] = {{256, 8, 4096}, {256, 16, 8192}, {512, 8, 8192}, {128, 16, 4096}, {256, 4, 2048}, {512, 4, 4096}};
     for (const auto& v : kVar) {
       tc_cfg cfg{};
       cfg.wg_size = v[0];

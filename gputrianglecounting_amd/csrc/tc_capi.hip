@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "tc.h"
 #include "tc_device.h"
@@ -42,10 +43,10 @@ int hip_fail(tc_ctx* c, hipError_t e, const char* where) {
 tc_cfg builtin_default() {
   tc_cfg c{};
   c.wg_size = 256;
-  c.items = 16;
+  c.items = 8;
   c.ring = 4096;
-  c.small_max = 4096;
-  c.wg_per_cu = 16;
+  c.small_max = 2048;
+  c.ranges_per_wg = 32;
   c.algo = 0;
   return c;
 }
@@ -59,7 +60,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->items) c.items = in->items;
     if (in->ring) c.ring = in->ring;
     if (in->small_max) c.small_max = in->small_max;
-    if (in->wg_per_cu) c.wg_per_cu = in->wg_per_cu;
+    if (in->ranges_per_wg) c.ranges_per_wg = in->ranges_per_wg;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -67,7 +68,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     tcd::Variant v = tcd::variant(i);
     if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring) vi = i;
   }
-  if (vi < 0 || c.wg_per_cu < 1 || c.wg_per_cu > 256 || c.small_max < 1 || c.algo < 0 || c.algo > 1) return -1;
+  if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 1) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   *out = c;
@@ -76,7 +77,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 
 // Scratch layout shared by the single-graph, batch and total-only paths.
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, scan, ooff, oind, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, total;
   int64_t ecap, ocap;
 };
 
@@ -101,6 +102,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
   L.ework = take(4 * (size_t)L.ecap);
   L.wpre = take(8 * (size_t)(L.ecap + 1));
   L.dscal = take(8 * 4);
+  L.qctr = take(4 * 8);
   const int64_t ncap = std::max<int64_t>(nv + 1, L.ecap);
   L.scan = take(8 * tcd::scan_tmp_elems(ncap));
   L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
@@ -141,6 +143,7 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.ework = reinterpret_cast<int32_t*>(b + L.ework);
   p.wpre = reinterpret_cast<int64_t*>(b + L.wpre);
   p.dscal = reinterpret_cast<int64_t*>(b + L.dscal);
+  p.qctr = reinterpret_cast<unsigned int*>(b + L.qctr);
   p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
   p.ecap = L.ecap;
   p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
@@ -152,8 +155,11 @@ void record(tc_ctx* c, int i, hipStream_t s) {
   if (c->timing && c->ev[i]) (void)hipEventRecord(c->ev[i], s);
 }
 
-int count_wg(const tc_ctx* c, const tc_cfg& cfg) {
-  int n = cfg.wg_per_cu * c->cus;
+// Persistent grid for the Merge Path kernel (every workgroup resident) and a
+// fixed grid for the one-lane-per-edge kernel.
+int count_wg(const tc_ctx* c, const tc_cfg& cfg, int vi) {
+  if (cfg.algo == 1) return 16 * c->cus;
+  int n = tcd::resident_wgs(vi, c->cus);
   if (n >= 64) n &= ~7;
   return std::max(n, 1);
 }
@@ -173,8 +179,9 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.total_mult = mult;
   a.small_max = cfg.small_max;
   a.min_units = 4 * cfg.wg_size * cfg.items;
-  const int nwg = count_wg(c, cfg);
-  a.nwg = nwg;
+  a.qctr = p.qctr;
+  const int nwg = count_wg(c, cfg, vi);
+  a.max_ranges = nwg * cfg.ranges_per_wg;
   hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, s);
   if (e != hipSuccess) return hip_fail(c, e, "count kernel launch");
   return TC_OK;
@@ -275,6 +282,14 @@ int tc_last_timing(tc_ctx* c, float ms[4]) {
 }
 
 int tc_device_cus(const tc_ctx* c) { return c ? c->cus : 0; }
+
+int tc_copy_i32(tc_ctx* c, int32_t* d_dst, const int32_t* d_src, int64_t n, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (n < 0 || (n > 0 && (!d_dst || !d_src))) return set_err(c, TC_EINVAL, "invalid argument");
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = tcd::launch_copy_i32(d_dst, d_src, n, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TC_OK : hip_fail(c, e, "copy");
+}
 
 int tc_last_edge_work(tc_ctx* c, int64_t out[2]) {
   if (!c || !out) return TC_EINVAL;
@@ -448,6 +463,265 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
   hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return TC_EHIP;
   return tc_count_i32(c, nv, nnz, d_off, d_ind, outPutTriangles, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ per-graph
+// baseline, HIP graphs and the autotuner
+struct tc_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+};
+
+namespace {
+
+hipStream_t capture_stream(tc_ctx* c) {
+  static thread_local hipStream_t s = nullptr;
+  static thread_local int dev = -1;
+  if (!s || dev != c->device) {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    dev = c->device;
+  }
+  return s;
+}
+
+// Runs fn(stream) under stream capture and instantiates the result.
+template <class F>
+int capture(tc_ctx* c, F&& fn, tc_graph** out) {
+  hipStream_t s = capture_stream(c);
+  if (!s) return set_err(c, TC_EHIP, "cannot create capture stream");
+  const bool timing = c->timing;
+  c->timing = false;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    c->timing = timing;
+    return hip_fail(c, e, "hipStreamBeginCapture");
+  }
+  const int rc = fn(s);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(s, &g);
+  c->timing = timing;
+  if (rc != TC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(c, e, "hipGraphInstantiate");
+  }
+  auto* r = new tc_graph;
+  r->graph = g;
+  r->exec = x;
+  r->device = c->device;
+  *out = r;
+  return TC_OK;
+}
+
+// Warm what a captured call must not do inside the capture: scratch
+// allocation and the occupancy query of the chosen kernel variant.
+int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, size_t bytes) {
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  (void)tcd::resident_wgs(vi, c->cus);
+  return ensure(c, bytes);
+}
+
+struct Candidate {
+  tc_cfg cfg;
+};
+
+std::vector<tc_cfg> tune_candidates() {
+  std::vector<tc_cfg> out;
+  for (int i = 0; i < tcd::num_variants(); ++i) {
+    const tcd::Variant v = tcd::variant(i);
+    const int lanecap = v.wg_size * v.items;
+    const int sms[3] = {256, 1024, lanecap};
+    for (int si = 0; si < 3; ++si) {
+      if (si < 2 && sms[si] >= lanecap) continue;
+      for (int rp : {8, 32, 128}) {
+        tc_cfg c{};
+        c.wg_size = v.wg_size;
+        c.items = v.items;
+        c.ring = v.ring;
+        c.small_max = sms[si];
+        c.ranges_per_wg = rp;
+        c.algo = 0;
+        out.push_back(c);
+      }
+    }
+  }
+  return out;
+}
+
+// run(cfg, stream) launches one count writing d_out; n = output length.
+template <class Run>
+int autotune(tc_ctx* c, Run&& run, int32_t* d_out, int64_t n, int reps, int set_default, tc_tune_result* results,
+             int n_results, int* n_written, tc_cfg* best) {
+  if (reps < 1) reps = 1;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  hipStream_t s = capture_stream(c);
+  if (!s) return set_err(c, TC_EHIP, "cannot create stream");
+  int32_t* ref = nullptr;
+  int32_t* flag = nullptr;
+  if ((e = hipMalloc(&ref, 4 * (size_t)std::max<int64_t>(n, 1))) != hipSuccess) return hip_fail(c, e, "hipMalloc");
+  if ((e = hipMalloc(&flag, 4)) != hipSuccess) {
+    (void)hipFree(ref);
+    return hip_fail(c, e, "hipMalloc");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const std::vector<tc_cfg> cands = tune_candidates();
+  float best_ms = 0;
+  int best_i = -1, written = 0, rc = TC_OK;
+  bool any_diff = false;
+  const bool timing = c->timing;
+  c->timing = false;
+  for (size_t i = 0; i < cands.size() && rc == TC_OK; ++i) {
+    rc = run(cands[i], s);
+    if (rc) break;
+    int32_t h = 0;
+    if (i == 0) {
+      e = hipMemcpyAsync(ref, d_out, 4 * (size_t)n, hipMemcpyDeviceToDevice, s);
+    } else {
+      e = tcd::launch_fill_i32(flag, 1, 0, s);
+      if (e == hipSuccess) e = tcd::launch_mismatch(ref, d_out, n, flag, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      rc = hip_fail(c, e, "autotune check");
+      break;
+    }
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == TC_OK; ++r) rc = run(cands[i], s);
+    (void)hipEventRecord(e1, s);
+    if (rc) break;
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+      rc = hip_fail(c, e, "autotune timing");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    const bool same = h == 0;
+    any_diff |= !same;
+    if (results && written < n_results) results[written++] = tc_tune_result{cands[i], ms, same ? 1 : 0};
+    if (same && (best_i < 0 || ms < best_ms)) {
+      best_i = (int)i;
+      best_ms = ms;
+    }
+  }
+  c->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(ref);
+  (void)hipFree(flag);
+  if (n_written) *n_written = written;
+  if (rc) return rc;
+  if (best_i >= 0) {
+    if (best) *best = cands[best_i];
+    if (set_default) c->def = cands[best_i];
+  }
+  if (any_diff) return set_err(c, TC_EGRAPH, "a tuned configuration produced different counts");
+  return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_batch_count_pergraph_i32(tc_ctx* c, int32_t G, const int64_t* h_tab, const int32_t* d_off_cat,
+                                const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals, const tc_cfg* cfg,
+                                void* stream) {
+  if (!c) return TC_EINVAL;
+  if (G <= 0 || !h_tab || !d_off_cat || !d_out_cat) return set_err(c, TC_EINVAL, "invalid argument");
+  // reserve for the largest graph so the loop allocates nothing
+  int64_t max_n = 0, max_nnz = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    max_n = std::max(max_n, h_tab[2 * g + 2] - h_tab[2 * g]);
+    max_nnz = std::max(max_nnz, h_tab[2 * g + 3] - h_tab[2 * g + 1]);
+  }
+  int rc = ensure(c, layout(max_n, max_nnz, false, false).total);
+  if (rc) return rc;
+  for (int32_t g = 0; g < G; ++g) {
+    const int64_t vb = h_tab[2 * g], ab = h_tab[2 * g + 1];
+    const int64_t n = h_tab[2 * g + 2] - vb, nnz = h_tab[2 * g + 3] - ab;
+    rc = tc_count_i32(c, (int32_t)n, nnz, d_off_cat + vb + g, d_ind_cat + ab, d_out_cat + vb,
+                      d_totals ? d_totals + g : nullptr, cfg, stream);
+    if (rc) return rc;
+  }
+  return TC_OK;
+}
+
+int tc_count_capture_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                         int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, tc_graph** out) {
+  if (!c || !out) return TC_EINVAL;
+  *out = nullptr;
+  int rc = prepare_capture(c, cfg, layout(nv, nnz, false, false).total);
+  if (rc) return rc;
+  return capture(
+      c, [&](hipStream_t s) { return tc_count_i32(c, nv, nnz, d_off, d_ind, d_out, d_total, cfg, s); }, out);
+}
+
+int tc_batch_capture_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,
+                         const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals,
+                         const tc_cfg* cfg, tc_graph** out) {
+  if (!c || !out) return TC_EINVAL;
+  *out = nullptr;
+  int rc = prepare_capture(c, cfg, layout(nv_total, nnz_total, true, false).total);
+  if (rc) return rc;
+  return capture(
+      c,
+      [&](hipStream_t s) {
+        return tc_batch_count_i32(c, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, d_totals, cfg,
+                                  s);
+      },
+      out);
+}
+
+int tc_graph_launch(tc_graph* g, void* stream) {
+  if (!g || !g->exec) return TC_EINVAL;
+  hipError_t e = hipSetDevice(g->device);
+  if (e == hipSuccess) e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TC_OK : TC_EHIP;
+}
+
+int tc_graph_destroy(tc_graph* g) {
+  if (!g) return TC_EINVAL;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return TC_OK;
+}
+
+int tc_autotune_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind, int32_t* d_out,
+                    int reps, int set_default, tc_tune_result* results, int n_results, int* n_written, tc_cfg* best) {
+  if (!c) return TC_EINVAL;
+  return autotune(
+      c, [&](const tc_cfg& cfg, hipStream_t s) { return tc_count_i32(c, nv, nnz, d_off, d_ind, d_out, nullptr, &cfg, s); },
+      d_out, nv, reps, set_default, results, n_results, n_written, best);
+}
+
+int tc_autotune_batch_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,
+                          const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int reps,
+                          int set_default, tc_tune_result* results, int n_results, int* n_written, tc_cfg* best) {
+  if (!c) return TC_EINVAL;
+  return autotune(
+      c,
+      [&](const tc_cfg& cfg, hipStream_t s) {
+        return tc_batch_count_i32(c, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, nullptr, &cfg, s);
+      },
+      d_out_cat, nv_total, reps, set_default, results, n_results, n_written, best);
 }
 
 }  // extern "C"

@@ -23,8 +23,9 @@ struct CountArgs {
   unsigned long long* total;    // int64 total (nullable)
   int32_t total_mult;           // total += total_mult * |N(u) ∩ N(w)| per edge
   int32_t small_max;
-  int32_t nwg;                  // workgroups launched
-  int32_t min_units;            // smallest level-1 range per workgroup
+  int32_t max_ranges;           // level-1 ranges: min(W / min_units, max_ranges)
+  int32_t min_units;            // smallest level-1 range
+  unsigned int* qctr;           // 8 per-XCD range queue counters, zeroed per launch
 };
 
 // Kernel variants compiled into the library.
@@ -45,6 +46,7 @@ struct Prep {
   int32_t* ework;   // [ecap]
   int64_t* wpre;    // [ecap + 1]
   int64_t* dscal;   // [4]
+  unsigned int* qctr;  // [8]
   int64_t* scan_tmp;
   int64_t ecap;
   // oriented copy for the total-only mode
@@ -64,10 +66,14 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
                               int cus, hipStream_t s);
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
+// Persistent grid: as many workgroups as fit on the device at once.
+int resident_wgs(int variant_idx, int cus);
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, hipStream_t s);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,
                            int cus, hipStream_t s);
+hipError_t launch_mismatch(const int32_t* a, const int32_t* b, int64_t n, int32_t* cnt, hipStream_t s);
+hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals,
                                hipStream_t s);
 

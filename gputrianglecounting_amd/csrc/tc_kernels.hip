@@ -314,309 +314,31 @@ __device__ __forceinline__ void level1_range(int64_t W, int nr, int lg, int64_t&
   R1 = R0 + q + (lg < r ? 1 : 0);
 }
 
-__device__ __forceinline__ int level1_parts(int64_t W, int nwg, int min_units) {
+__device__ __forceinline__ int level1_parts(int64_t W, int max_ranges, int min_units) {
   int64_t nr = W / (min_units > 0 ? min_units : 1);
   if (nr < 1) nr = 1;
-  if (nr > nwg) nr = nwg;
+  if (nr > max_ranges) nr = max_ranges;
   return (int)nr;
 }
 
-// Two-level Merge Path count.
-//   BS  threads per workgroup, K merge steps per lane per tile,
-//   P   LDS ring length per list (power of two), M max edges per small tile.
-// A workgroup walks its range of the concatenated Merge Paths of all edges
-// tile by tile.  Ring tiles (one long edge, or a partial edge at a range
-// boundary) stream both lists through two LDS rings so every element is
-// loaded from memory once per pass; small tiles pack up to M whole short
-// edges, each staged as [N(u) | INT_MAX | N(w) | INT_MAX].  In both, lane l
-// takes diagonals [l*K, l*K + K) of the tile, finds its start with a
-// binary search on the staged lists and merges serially.  A match
-// A[x] == B[y] is counted by the lane that consumes A[x] (ties go to A, so
-// B[y] is then the head of B): every match is counted exactly once, with no
-// hand-off between lanes or workgroups.
-template <int BS, int K, int P, int M>
-__global__ __launch_bounds__(BS) void k_count_mp(CountArgs a) {
-  constexpr int NW = BS / 64;
-  constexpr int LANECAP = BS * K;
-  constexpr int RMASK = P - 1;
-  constexpr int TMAX = P - 2;
-  static_assert((P & (P - 1)) == 0, "ring length must be a power of two");
-  static_assert(LANECAP >= TMAX, "lanes must cover a ring tile");
-  static_assert(LANECAP + 2 * M <= 2 * P, "a small tile must fit the LDS buffer");
-  static_assert(M <= BS, "one metadata slot per thread");
-
-  __shared__ int32_t buf[2 * P];
-  __shared__ int32_t m_ws[M + 1];
-  __shared__ int32_t m_la[M], m_lb[M], m_cnt[M], m_u[M];
-  __shared__ int64_t m_ag[M], m_bg[M];
-  __shared__ int64_t red64[NW];
-  __shared__ int32_t red32[NW];
-  __shared__ int32_t s_ij[2];
-  __shared__ int64_t s_e;
-
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t E = a.dscal[0], W = a.dscal[1];
-  const int nr = level1_parts(W, a.nwg, a.min_units);
-  int lg = blockIdx.x;
-  if (nr == a.nwg && (a.nwg & 7) == 0) lg = (blockIdx.x & 7) * (a.nwg >> 3) + (blockIdx.x >> 3);
-  if (lg >= nr) return;
-  int64_t R0, R1;
-  level1_range(W, nr, lg, R0, R1);
-  if (R0 >= R1) return;
-
-  if (tid == 0) {
-    int64_t lo = 0, hi = E + 1;  // first index with wpre > R0
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (a.wpre[mid] <= R0) lo = mid + 1;
-      else hi = mid;
-    }
-    const int64_t e = lo - 1;
-    const int64_t t = R0 - a.wpre[e];
-    int32_t i = 0;
-    if (t > 0) {
-      const int32_t u = a.eu[e], w = a.ew[e];
-      const int64_t ab = a.voff[u], bb = a.voff[w];
-      const int64_t la = a.voff[u + 1] - ab, lb = a.voff[w + 1] - bb;
-      int64_t l2 = t - lb > 0 ? t - lb : 0, h2 = t < la ? t : la;
-      while (l2 < h2) {
-        int64_t mid = (l2 + h2) >> 1;
-        if (a.ind[ab + mid] <= a.ind[bb + t - mid - 1]) l2 = mid + 1;
-        else h2 = mid;
-      }
-      i = (int32_t)l2;
-    }
-    s_e = e;
-    s_ij[0] = i;
-    s_ij[1] = (int32_t)(t - i);
+// Next level-1 range for a workgroup on XCD xcd.  The nr ranges are cut into
+// 8 contiguous blocks, one queue per XCD (workgroup i runs on XCD i % 8), so
+// each XCD's L2 sees a contiguous part of the edge list; a workgroup whose
+// queue is empty takes ranges from the other XCDs' queues.  -1 when all
+// ranges are taken.
+__device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
+  for (int k = 0; k < 8; ++k) {
+    const int x = (xcd + k) & 7;
+    const int b0 = (int)(((int64_t)nr * x) >> 3), b1 = (int)(((int64_t)nr * (x + 1)) >> 3);
+    if (b1 <= b0) continue;
+    if (__hip_atomic_load(&q[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)(b1 - b0)) continue;
+    const unsigned r = atomicAdd(&q[x], 1u);
+    if (r < (unsigned)(b1 - b0)) return b0 + (int)r;
   }
-  __syncthreads();
-  int64_t e = s_e;
-  int32_t ci = s_ij[0], cj = s_ij[1];
-  int64_t pos = R0;
-  int64_t my_sum = 0;  // sum of edge counts flushed by this thread
-  __syncthreads();
-
-  while (pos < R1) {
-    const int64_t we0 = a.wpre[e], we1 = a.wpre[e + 1];
-    const int64_t L = we1 - we0;
-    const int64_t done = (int64_t)ci + cj;
-    const int64_t rem = L - done;
-    const bool ring = done > 0 || rem > a.small_max || (R1 - pos) < rem;
-    if (ring) {
-      // ---------------- one edge through the LDS rings
-      const int64_t units = rem < (R1 - pos) ? rem : (R1 - pos);
-      const int32_t u = a.eu[e], w = a.ew[e];
-      const int64_t ab = a.voff[u], bb = a.voff[w];
-      const int32_t la = (int32_t)(a.voff[u + 1] - ab), lb = (int32_t)(a.voff[w + 1] - bb);
-      int32_t a_hi = ci, b_hi = cj;
-      int32_t cnt = 0;
-      int64_t left = units;
-      while (left > 0) {
-        const int32_t T = (int32_t)(left < TMAX ? left : TMAX);
-        const int32_t a_to = ci + T < la ? ci + T : la;
-        const int32_t b_to = cj + T < lb ? cj + T : lb;
-        for (int32_t x = (a_hi > ci ? a_hi : ci) + tid; x < a_to; x += BS) buf[x & RMASK] = a.ind[ab + x];
-        for (int32_t y = (b_hi > cj ? b_hi : cj) + tid; y < b_to; y += BS) buf[P + (y & RMASK)] = a.ind[bb + y];
-        if (tid == 0) {
-          if (a_to == la) buf[la & RMASK] = INT_MAX;
-          if (b_to == lb) buf[P + (lb & RMASK)] = INT_MAX;
-        }
-        a_hi = a_hi > a_to ? a_hi : a_to;
-        b_hi = b_hi > b_to ? b_hi : b_to;
-        __syncthreads();
-        const int32_t d = tid * K;
-        if (d < T) {
-          int32_t l2 = d - (lb - cj);
-          l2 = l2 > 0 ? l2 : 0;
-          int32_t h2 = d < la - ci ? d : la - ci;
-          while (l2 < h2) {
-            const int32_t mid = (l2 + h2) >> 1;
-            const int32_t av = buf[(ci + mid) & RMASK];
-            const int32_t bv = buf[P + ((cj + d - mid - 1) & RMASK)];
-            if (av <= bv) l2 = mid + 1;
-            else h2 = mid;
-          }
-          int32_t ia = ci + l2, jb = cj + d - l2;
-          int32_t av = buf[ia & RMASK], bv = buf[P + (jb & RMASK)];
-          const int32_t steps = K < T - d ? K : T - d;
-          if (steps == K) {
-#pragma unroll
-            for (int s = 0; s < K; ++s) {
-              const bool ta = av <= bv;
-              cnt += (av == bv);
-              ia += ta;
-              jb += !ta;
-              const int32_t nvv = buf[ta ? (ia & RMASK) : (P + (jb & RMASK))];
-              av = ta ? nvv : av;
-              bv = ta ? bv : nvv;
-            }
-          } else {
-            for (int s = 0; s < steps; ++s) {
-              const bool ta = av <= bv;
-              cnt += (av == bv);
-              ia += ta;
-              jb += !ta;
-              const int32_t nvv = buf[ta ? (ia & RMASK) : (P + (jb & RMASK))];
-              av = ta ? nvv : av;
-              bv = ta ? bv : nvv;
-            }
-          }
-          if (d + steps == T) {
-            s_ij[0] = ia;
-            s_ij[1] = jb;
-          }
-        }
-        __syncthreads();
-        ci = s_ij[0];
-        cj = s_ij[1];
-        left -= T;
-      }
-      const int64_t c = block_sum64<BS>(cnt, red64);
-      if (tid == 0 && c) {
-        if (a.out) {
-          atomicAdd(&a.out[u], (int32_t)c);
-          atomicAdd(&a.out[w], (int32_t)c);
-        }
-        my_sum += c;
-      }
-      pos += units;
-      if (units == rem) {
-        ++e;
-        ci = 0;
-        cj = 0;
-      }
-    } else {
-      // ---------------- several whole short edges
-      bool ok = false;
-      const int64_t ee = e + tid;
-      int32_t cw = 0, wk = 0;
-      if (tid < M && ee < E) {
-        const int64_t w1 = a.wpre[ee + 1];
-        const int64_t cw64 = w1 - we0;
-        const int64_t wk64 = w1 - a.wpre[ee];
-        ok = cw64 <= LANECAP && cw64 + 2 * (tid + 1) <= 2 * P && wk64 <= a.small_max && w1 <= R1;
-        cw = (int32_t)(cw64 < INT_MAX ? cw64 : INT_MAX);
-        wk = (int32_t)(wk64 < INT_MAX ? wk64 : INT_MAX);
-      }
-      const uint64_t bad = __ballot(!ok);
-      if (lane == 0) red32[wid] = bad ? (wid * 64 + __builtin_ctzll(bad)) : BS;
-      __syncthreads();
-      int32_t c = BS;
-#pragma unroll
-      for (int q = 0; q < NW; ++q) c = red32[q] < c ? red32[q] : c;
-      if (tid < c) {
-        const int32_t u = a.eu[ee], w = a.ew[ee];
-        const int64_t ag = a.voff[u], bg = a.voff[w];
-        m_ws[tid] = cw - wk;
-        m_la[tid] = (int32_t)(a.voff[u + 1] - ag);
-        m_lb[tid] = (int32_t)(a.voff[w + 1] - bg);
-        m_cnt[tid] = 0;
-        m_u[tid] = u;
-        m_ag[tid] = ag;
-        m_bg[tid] = bg;
-        if (tid == c - 1) m_ws[c] = cw;
-      }
-      __syncthreads();
-      const int32_t cum = m_ws[c];
-      {
-        // stage with a power-of-two group of threads per edge
-        const int32_t per = BS / c;
-        const int32_t gsh = 31 - __builtin_clz((unsigned)per);
-        const int32_t k = tid >> gsh, r = tid & ((1 << gsh) - 1), g = 1 << gsh;
-        if (k < c) {
-          const int32_t la = m_la[k], lb = m_lb[k], base = m_ws[k] + 2 * k;
-          const int64_t ag = m_ag[k], bg = m_bg[k];
-          for (int32_t p = r; p < la; p += g) buf[base + p] = a.ind[ag + p];
-          for (int32_t p = r; p < lb; p += g) buf[base + la + 1 + p] = a.ind[bg + p];
-          if (r == 0) {
-            buf[base + la] = INT_MAX;
-            buf[base + la + 1 + lb] = INT_MAX;
-          }
-        }
-      }
-      __syncthreads();
-      const int32_t d0 = tid * K;
-      if (d0 < cum) {
-        int32_t l2 = 0, h2 = c;  // first k with m_ws[k] > d0
-        while (l2 < h2) {
-          const int32_t mid = (l2 + h2) >> 1;
-          if (m_ws[mid] <= d0) l2 = mid + 1;
-          else h2 = mid;
-        }
-        int32_t k = l2 - 1;
-        int32_t left = K < cum - d0 ? K : cum - d0;
-        int32_t d = d0 - m_ws[k];
-        int32_t la = m_la[k], lb = m_lb[k];
-        int32_t abase = m_ws[k] + 2 * k, bbase = abase + la + 1;
-        int32_t x0 = d - lb > 0 ? d - lb : 0, x1 = d < la ? d : la;
-        while (x0 < x1) {
-          const int32_t mid = (x0 + x1) >> 1;
-          if (buf[abase + mid] <= buf[bbase + d - mid - 1]) x0 = mid + 1;
-          else x1 = mid;
-        }
-        int32_t ia = x0, jb = d - x0;
-        int32_t av = buf[abase + ia], bv = buf[bbase + jb];
-        int32_t cnt = 0;
-        for (;;) {
-          const int32_t room = la + lb - d;
-          const int32_t n = left < room ? left : room;
-          for (int32_t s = 0; s < n; ++s) {
-            const bool ta = av <= bv;
-            cnt += (av == bv);
-            ia += ta;
-            jb += !ta;
-            const int32_t nvv = buf[ta ? abase + ia : bbase + jb];
-            av = ta ? nvv : av;
-            bv = ta ? bv : nvv;
-          }
-          left -= n;
-          d += n;
-          if (d == la + lb) {
-            if (cnt) atomicAdd(&m_cnt[k], cnt);
-            cnt = 0;
-            if (left == 0) break;
-            ++k;
-            la = m_la[k];
-            lb = m_lb[k];
-            abase = m_ws[k] + 2 * k;
-            bbase = abase + la + 1;
-            d = 0;
-            ia = 0;
-            jb = 0;
-            av = buf[abase];
-            bv = buf[bbase];
-          } else {
-            break;
-          }
-        }
-        if (cnt) atomicAdd(&m_cnt[k], cnt);
-      }
-      __syncthreads();
-      if (tid < c) {
-        const int32_t cc = m_cnt[tid];
-        if (cc) {
-          if (a.out) atomicAdd(&a.out[a.ew[ee]], cc);
-          my_sum += cc;
-        }
-        // u side: the first edge of each run of equal u adds the run's sum
-        const int32_t u = m_u[tid];
-        if (a.out && (tid == 0 || m_u[tid - 1] != u)) {
-          int32_t s = 0;
-          for (int32_t t2 = tid; t2 < c && m_u[t2] == u; ++t2) s += m_cnt[t2];
-          if (s) atomicAdd(&a.out[u], s);
-        }
-      }
-      __syncthreads();
-      pos += cum;
-      e += c;
-      ci = 0;
-      cj = 0;
-    }
-  }
-  const int64_t tsum = block_sum64<BS>(my_sum, red64);
-  if (tid == 0 && a.total && tsum) atomicAdd(a.total, (unsigned long long)(tsum * a.total_mult));
+  return -1;
 }
+
+#include "tc_mp_kernel.h"
 
 // One lane per edge, two-pointer merge from global memory.  Simple and
 // unbalanced; kept as an independent cross-check of k_count_mp.
@@ -656,6 +378,25 @@ __global__ void k_lcc(const int32_t* __restrict__ off, int32_t nv, const int32_t
     const int64_t d = (int64_t)off[v + 1] - off[v];
     lcc[v] = d < 2 ? 0.0 : (double)out[v] / (double)(d * (d - 1));
   }
+}
+
+// Plain copy, 16 bytes per lane per iteration (counter calibration).
+__global__ __launch_bounds__(256) void k_copy_i32(int4* __restrict__ dst, const int4* __restrict__ src, int64_t n4,
+                                                  int32_t* __restrict__ dtail, const int32_t* __restrict__ stail,
+                                                  int32_t ntail) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t i = g; i < n4; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+  if (g < ntail) dtail[g] = stail[g];
+}
+
+// Number of positions where a and b differ (autotuner output check).
+__global__ __launch_bounds__(256) void k_mismatch(const int32_t* __restrict__ a, const int32_t* __restrict__ b,
+                                                  int64_t n, int32_t* __restrict__ cnt) {
+  int32_t c = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += a[i] != b[i];
+  c = wave_sum32(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
 }
 
 // Per-graph totals of a batch: one block per graph sums its vertex range.
@@ -723,11 +464,12 @@ struct VariantDef {
   void (*fn)(CountArgs);
 };
 const VariantDef kVariants[] = {
-    {256, 16, 4096, 128, k_count_mp<256, 16, 4096, 128>},
-    {256, 8, 2048, 128, k_count_mp<256, 8, 2048, 128>},
-    {512, 8, 4096, 128, k_count_mp<512, 8, 4096, 128>},
-    {256, 32, 8192, 128, k_count_mp<256, 32, 8192, 128>},
-    {128, 32, 4096, 64, k_count_mp<128, 32, 4096, 64>},
+    {256, 8, 4096, 128, k_count_mp<256, 8, 4096, 128>},
+    {256, 16, 8192, 128, k_count_mp<256, 16, 8192, 128>},
+    {512, 8, 8192, 128, k_count_mp<512, 8, 8192, 128>},
+    {128, 16, 4096, 64, k_count_mp<128, 16, 4096, 64>},
+    {256, 4, 2048, 64, k_count_mp<256, 4, 2048, 64>},
+    {512, 4, 4096, 128, k_count_mp<512, 4, 4096, 128>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -798,15 +540,28 @@ hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus,
   return hipGetLastError();
 }
 
+int resident_wgs(int vi, int cus) {
+  static int occ[kNumVariants] = {0};
+  if (vi < 0 || vi >= kNumVariants) return 0;
+  if (!occ[vi]) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(kVariants[vi].fn),
+                                                     kVariants[vi].bs, 0) != hipSuccess || n < 1)
+      n = 1;
+    occ[vi] = n;
+  }
+  return occ[vi] * cus;
+}
+
 hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, hipStream_t s) {
   if (algo == 1) {
     hipLaunchKernelGGL(k_count_simple, dim3(nwg), dim3(256), 0, s, a);
     return hipGetLastError();
   }
   if (vi < 0 || vi >= kNumVariants) return hipErrorInvalidValue;
-  CountArgs b = a;
-  b.nwg = nwg;
-  hipLaunchKernelGGL(kVariants[vi].fn, dim3(nwg), dim3(kVariants[vi].bs), 0, s, b);
+  hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.qctr), 8, 0, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kVariants[vi].fn, dim3(nwg), dim3(kVariants[vi].bs), 0, s, a);
   return hipGetLastError();
 }
 
@@ -820,6 +575,22 @@ hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, i
                            hipStream_t s) {
   hipLaunchKernelGGL(k_validate, dim3(grid_for((int64_t)nv * 64 + 64, 256, (int64_t)cus * 32)), dim3(256), 0, s,
                      off, ind, nv, nnz, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_mismatch(const int32_t* a, const int32_t* b, int64_t n, int32_t* cnt, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mismatch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b, n, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t n4 = n / 4;
+  const int32_t nt = (int32_t)(n - 4 * n4);
+  hipLaunchKernelGGL(k_copy_i32, dim3(grid_for(n4 > 0 ? n4 : 1, 256, 16384)), dim3(256), 0, s,
+                     reinterpret_cast<int4*>(dst), reinterpret_cast<const int4*>(src), n4, dst + 4 * n4,
+                     src + 4 * n4, nt);
   return hipGetLastError();
 }
 

@@ -35,7 +35,7 @@ class TcCfg(C.Structure):
     """Mirror of ``tc_cfg`` (include/tc.h)."""
 
     _fields_ = [("wg_size", C.c_int32), ("items", C.c_int32), ("ring", C.c_int32),
-                ("small_max", C.c_int32), ("wg_per_cu", C.c_int32), ("algo", C.c_int32)]
+                ("small_max", C.c_int32), ("ranges_per_wg", C.c_int32), ("algo", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -46,6 +46,35 @@ class TcCfg(C.Structure):
         for k, v in kw.items():
             setattr(c, k, int(v))
         return c
+
+
+class TuneResult(C.Structure):
+    """Mirror of ``tc_tune_result``."""
+
+    _fields_ = [("cfg", TcCfg), ("ms", C.c_float), ("identical", C.c_int32)]
+
+
+class CapturedCount:
+    """A recorded count call (``tc_graph``): replay with :meth:`launch`."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def launch(self, stream=None) -> None:
+        rc = lib().tc_graph_launch(self._h, _stream(stream))
+        if rc != TC_OK:
+            raise TCError("tc_graph_launch failed", rc)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().tc_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def lib() -> C.CDLL:
@@ -77,6 +106,16 @@ def lib() -> C.CDLL:
         "tc_count_total_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
         "tc_device_cus": (C.c_int, [vp]),
         "tc_last_edge_work": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "tc_copy_i32": (C.c_int, [vp, vp, vp, i64, vp]),
+        "tc_batch_count_pergraph_i32": (C.c_int, [vp, i32, C.POINTER(C.c_int64), vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_count_capture_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), C.POINTER(vp)]),
+        "tc_batch_capture_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), C.POINTER(vp)]),
+        "tc_graph_launch": (C.c_int, [vp, vp]),
+        "tc_graph_destroy": (C.c_int, [vp]),
+        "tc_autotune_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, C.c_int, C.c_int, C.POINTER(TuneResult), C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(TcCfg)]),
+        "tc_autotune_batch_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, C.c_int, C.c_int,
+                                            C.POINTER(TuneResult), C.c_int, C.POINTER(C.c_int), C.POINTER(TcCfg)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -90,7 +129,9 @@ def lib() -> C.CDLL:
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
            "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
-           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus", "tc_last_edge_work"]
+           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
+           "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
+           "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32"]
 
 
 def _torch():
@@ -254,6 +295,51 @@ class TriangleCounter:
                                       _p(totals), _cfgp(cfg), _stream(stream))
         self._check(rc, "tc_batch_count_i32")
         return out, totals
+
+    def batch_count_pergraph(self, tab_host, off_cat, ind_cat, out, totals=None, cfg=None, stream=None):
+        """One tc_count_i32 per graph (the per-graph-launch baseline)."""
+        tab_host = np.ascontiguousarray(tab_host, dtype=np.int64)
+        G = tab_host.shape[0] // 2 - 1
+        rc = lib().tc_batch_count_pergraph_i32(self._h, G, tab_host.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               _p(off_cat), _p(ind_cat), _p(out), _p(totals), _cfgp(cfg),
+                                               _stream(stream))
+        self._check(rc, "tc_batch_count_pergraph_i32")
+        return out, totals
+
+    def capture_count(self, off, ind, out, total, cfg=None) -> CapturedCount:
+        h = C.c_void_p()
+        nv, nnz = int(off.shape[0]) - 1, int(ind.shape[0])
+        rc = lib().tc_count_capture_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(out), _p(total),
+                                        _cfgp(cfg), C.byref(h))
+        self._check(rc, "tc_count_capture_i32")
+        return CapturedCount(h)
+
+    def capture_batch(self, tab, off_cat, ind_cat, nv_total, nnz_total, out, totals, cfg=None) -> CapturedCount:
+        h = C.c_void_p()
+        G = int(tab.shape[0]) // 2 - 1
+        rc = lib().tc_batch_capture_i32(self._h, G, _p(tab), nv_total, nnz_total, _p(off_cat), _p(ind_cat),
+                                        _p(out), _p(totals), _cfgp(cfg), C.byref(h))
+        self._check(rc, "tc_batch_capture_i32")
+        return CapturedCount(h)
+
+    def _tune(self, fn, args, reps, set_default):
+        res = (TuneResult * 256)()
+        nw = C.c_int(0)
+        best = TcCfg()
+        rc = fn(self._h, *args, reps, 1 if set_default else 0, res, 256, C.byref(nw), C.byref(best))
+        rows = [dict(r.cfg.as_dict(), ms=float(r.ms), identical=bool(r.identical)) for r in res[: nw.value]]
+        self._check(rc, "autotune")
+        return best.as_dict(), rows
+
+    def autotune(self, off, ind, out, reps=3, set_default=True):
+        """Time every candidate configuration; returns (best cfg, table)."""
+        nv, nnz = int(off.shape[0]) - 1, int(ind.shape[0])
+        return self._tune(lib().tc_autotune_i32, (nv, nnz, _p(off), _p(ind), _p(out)), reps, set_default)
+
+    def autotune_batch(self, tab, off_cat, ind_cat, nv_total, nnz_total, out, reps=3, set_default=True):
+        G = int(tab.shape[0]) // 2 - 1
+        return self._tune(lib().tc_autotune_batch_i32,
+                          (G, _p(tab), nv_total, nnz_total, _p(off_cat), _p(ind_cat), _p(out)), reps, set_default)
 
     # ------------------------------------------------------------ host convenience
     def to_device(self, g: Graph):

@@ -43,8 +43,9 @@ typedef struct tc_ctx tc_ctx;
  *   small_max      edges with at most this much work (|N(u)|+|N(w)|) are
  *                  packed several to a tile; longer ones stream through the
  *                  LDS rings
- *   wg_per_cu      level-1 split: the edge-work range is cut into
- *                  wg_per_cu * CUs equal parts (one per workgroup)
+ *   ranges_per_wg  level-1 split: the edge-work range [0, W) is cut into
+ *                  ranges_per_wg * (resident workgroups) equal ranges that a
+ *                  persistent grid pulls from per-XCD queues
  *   algo           0 = two-level Merge Path (default), 1 = one lane per
  *                  edge (simple, for cross-checks) */
 typedef struct tc_cfg {
@@ -52,7 +53,7 @@ typedef struct tc_cfg {
   int32_t items;
   int32_t ring;
   int32_t small_max;
-  int32_t wg_per_cu;
+  int32_t ranges_per_wg;
   int32_t algo;
 } tc_cfg;
 
@@ -132,6 +133,52 @@ int tc_batch_count_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t h_n
                        const tc_cfg* cfg, void* stream);
 size_t tc_batch_scratch_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
 
+/* The same G graphs counted one tc_count_i32 call per graph, in order, on
+ * one stream (the plain path, one launch sequence per graph): the baseline
+ * the batched launch is compared against.  h_tab is the host copy of
+ * d_tab's layout; d_totals (nullable) gets each graph's total. */
+int tc_batch_count_pergraph_i32(tc_ctx* ctx, int32_t G, const int64_t* h_tab, const int32_t* d_off_cat,
+                                const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals, const tc_cfg* cfg,
+                                void* stream);
+
+/* ---------------------------------------------------------------- graphs
+ * One count call recorded as a HIP graph, replayable on any stream with no
+ * host-side work and no allocation: scratch is reserved before recording,
+ * every address is fixed, and buffers are cleared by fill kernels (no
+ * memset nodes).  Input contents may change between replays; sizes and
+ * addresses may not.  No reference counterpart (the reference launches
+ * from the host each time, clusteringCount.cu:425-463). */
+typedef struct tc_graph tc_graph;
+int tc_count_capture_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                         int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, tc_graph** out);
+int tc_batch_capture_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t h_nv_total, int64_t h_nnz_total,
+                         const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals,
+                         const tc_cfg* cfg, tc_graph** out);
+int tc_graph_launch(tc_graph* g, void* stream);
+int tc_graph_destroy(tc_graph* g);
+
+/* ---------------------------------------------------------------- autotuner
+ * Candidates: every compiled kernel variant (wg_size, items, ring) times
+ * small_max in {256, 1024, wg_size*items} times ranges_per_wg in
+ * {8, 32, 128}.  Each candidate runs once, its per-vertex output is
+ * compared with the first candidate's (identical must hold: the counts are
+ * exact integers), then it is timed over reps launches with HIP events.
+ * The fastest identical candidate is returned in *best and, when
+ * set_default is nonzero, becomes the context default.  Up to n_results
+ * entries are written to results (nullable).  Returns TC_EGRAPH if any
+ * candidate's output differed. */
+typedef struct tc_tune_result {
+  tc_cfg cfg;
+  float ms;
+  int32_t identical;
+} tc_tune_result;
+int tc_autotune_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                    int32_t* d_out, int reps, int set_default, tc_tune_result* results, int n_results,
+                    int* n_written, tc_cfg* best);
+int tc_autotune_batch_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t h_nv_total, int64_t h_nnz_total,
+                          const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int reps,
+                          int set_default, tc_tune_result* results, int n_results, int* n_written, tc_cfg* best);
+
 /* Global-count mode (README.md:6-10): only the int64 total 6·T, via a
  * degree-ordered orientation so each triangle is intersected once. */
 int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
@@ -141,6 +188,12 @@ int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_of
  * |N(u)| + |N(w)| of the last count on this context (synchronises the
  * context's device).  out[0] = E, out[1] = W. */
 int tc_last_edge_work(tc_ctx* ctx, int64_t out[2]);
+
+/* Plain device copy d_dst[i] = d_src[i] of n int32 values (one read and one
+ * write per element, 16-byte vector accesses).  No reference counterpart:
+ * it calibrates hardware byte counters (rocprofv3 FETCH_SIZE / WRITE_SIZE)
+ * against a transfer of known size. */
+int tc_copy_i32(tc_ctx* ctx, int32_t* d_dst, const int32_t* d_src, int64_t n, void* stream);
 
 /* Number of compute units of the context's device (for sizing). */
 int tc_device_cus(const tc_ctx* ctx);

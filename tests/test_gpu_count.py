@@ -14,7 +14,7 @@ from tests import graphs as GR
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(256, 16, 4096), (256, 8, 2048), (512, 8, 4096), (256, 32, 8192), (128, 32, 4096)]
+VARIANTS = [(256, 8, 4096), (256, 16, 8192), (512, 8, 8192), (128, 16, 4096), (256, 4, 2048), (512, 4, 4096)]
 
 
 def _check(counter, g, cfg=None):
@@ -38,10 +38,10 @@ def test_random_graphs(counter, g):
 
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: "x".join(map(str, v)))
 @pytest.mark.parametrize("small_max", [16, 300, 0])
-@pytest.mark.parametrize("wg_per_cu", [1, 16])
-def test_variants(counter, v, small_max, wg_per_cu):
+@pytest.mark.parametrize("ranges", [1, 32])
+def test_variants(counter, v, small_max, ranges):
     g = G.rmat(13, 16, seed=21)
-    cfg = dict(wg_size=v[0], items=v[1], ring=v[2], small_max=small_max, wg_per_cu=wg_per_cu)
+    cfg = dict(wg_size=v[0], items=v[1], ring=v[2], small_max=small_max, ranges_per_wg=ranges)
     _check(counter, g, cfg)
 
 
@@ -132,4 +132,4 @@ def test_errors(counter):
     with pytest.raises(T.TCError):
         counter.count(off, ind, cfg=dict(wg_size=100))
     with pytest.raises(T.TCError):
-        counter.count(off, ind, cfg=dict(wg_per_cu=1000))
+        counter.count(off, ind, cfg=dict(ranges_per_wg=100000))
