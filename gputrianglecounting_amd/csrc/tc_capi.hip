@@ -66,9 +66,10 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
   int vi = -1;
   for (int i = 0; i < tcd::num_variants(); ++i) {
     tcd::Variant v = tcd::variant(i);
-    if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring) vi = i;
+    const int want = c.algo == 1 ? 0 : c.algo;  // the simple kernel reuses variant 0's sizes
+    if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want) vi = i;
   }
-  if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 1) return -1;
+  if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 2) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   *out = c;
@@ -542,6 +543,7 @@ std::vector<tc_cfg> tune_candidates() {
   std::vector<tc_cfg> out;
   for (int i = 0; i < tcd::num_variants(); ++i) {
     const tcd::Variant v = tcd::variant(i);
+    if (v.algo != 0) continue;  // the tuner searches the default (streamed) kernel
     const int lanecap = v.wg_size * v.items;
     const int sms[3] = {256, 1024, lanecap};
     for (int si = 0; si < 3; ++si) {
@@ -553,7 +555,7 @@ std::vector<tc_cfg> tune_candidates() {
         c.ring = v.ring;
         c.small_max = sms[si];
         c.ranges_per_wg = rp;
-        c.algo = 0;
+        c.algo = v.algo;
         out.push_back(c);
       }
     }

@@ -30,7 +30,7 @@ struct CountArgs {
 
 // Kernel variants compiled into the library.
 struct Variant {
-  int wg_size, items, ring;
+  int wg_size, items, ring, algo;
 };
 int num_variants();
 Variant variant(int i);

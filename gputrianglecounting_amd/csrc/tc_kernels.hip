@@ -340,6 +340,68 @@ __device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
 
 #include "tc_mp_kernel.h"
 
+// Count the elements of chunks [c_lo, c_hi) of a 16-byte aligned array base
+// (chunk c = elements [c*CH, c*CH + CH)), restricted to element positions
+// [g0, g1), that occur in the sorted list A[0, lA) held in LDS with
+// A[lA] = INT_MAX.  Chunk c+1 is loaded while chunk c is compared.
+template <int CH>
+__device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
+                                                 int64_t g0, int64_t g1, const int32_t* A, int32_t lA) {
+  constexpr int Q = CH / 4;
+  if (c_lo >= c_hi) return 0;
+  const int64_t gf = c_lo * CH > g0 ? c_lo * CH : g0;
+  const int32_t bf = base[gf];
+  int32_t lo = 0, hi = lA;  // lower bound of bf in A
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (A[mid] < bf) lo = mid + 1;
+    else hi = mid;
+  }
+  int32_t pos = lo;
+  int32_t acur = A[pos];
+  int32_t cnt = 0;
+  const int4* b4 = reinterpret_cast<const int4*>(base);
+  int4 cur[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) cur[q] = b4[c_lo * Q + q];
+  for (int64_t c = c_lo; c < c_hi; ++c) {
+    int4 nxt[Q];
+    if (c + 1 < c_hi) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) nxt[q] = b4[(c + 1) * Q + q];
+    }
+    int32_t b[CH];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      b[4 * q] = cur[q].x;
+      b[4 * q + 1] = cur[q].y;
+      b[4 * q + 2] = cur[q].z;
+      b[4 * q + 3] = cur[q].w;
+    }
+    const int64_t cbase = c * CH;
+    if (cbase < g0 || cbase + CH > g1) {
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int64_t g = cbase + k;
+        if (g < g0 || g >= g1) b[k] = -1;  // ids are >= 0: never matches, never advances
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int32_t bv = b[k];
+      while (acur < bv) acur = A[++pos];
+      cnt += (acur == bv);
+    }
+    if (c + 1 < c_hi) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) cur[q] = nxt[q];
+    }
+  }
+  return cnt;
+}
+
+#include "tc_bs_kernel.h"
+
 // One lane per edge, two-pointer merge from global memory.  Simple and
 // unbalanced; kept as an independent cross-check of k_count_mp.
 __global__ __launch_bounds__(256) void k_count_simple(CountArgs a) {
@@ -458,25 +520,30 @@ unsigned grid_for(int64_t n, int bs, int64_t cap = 65536) {
   return (unsigned)g;
 }
 
-// Compiled variants of k_count_mp: {BS, K, P, M}.
+// Compiled count kernels: algo 0 = streamed (k_count_bs), 2 = Merge Path
+// rings only (k_count_mp).  {algo, BS, K, P, M}.
 struct VariantDef {
-  int bs, k, p, m;
+  int algo, bs, k, p, m;
   void (*fn)(CountArgs);
 };
 const VariantDef kVariants[] = {
-    {256, 8, 4096, 128, k_count_mp<256, 8, 4096, 128>},
-    {256, 16, 8192, 128, k_count_mp<256, 16, 8192, 128>},
-    {512, 8, 8192, 128, k_count_mp<512, 8, 8192, 128>},
-    {128, 16, 4096, 64, k_count_mp<128, 16, 4096, 64>},
-    {256, 4, 2048, 64, k_count_mp<256, 4, 2048, 64>},
-    {512, 4, 4096, 128, k_count_mp<512, 4, 4096, 128>},
+    {0, 256, 8, 4096, 128, k_count_bs<256, 16, 4096, 128, 8>},
+    {0, 256, 4, 2048, 64, k_count_bs<256, 16, 2048, 64, 4>},
+    {0, 512, 4, 4096, 128, k_count_bs<512, 16, 4096, 128, 4>},
+    {0, 128, 16, 4096, 64, k_count_bs<128, 16, 4096, 64, 16>},
+    {2, 256, 8, 4096, 128, k_count_mp<256, 8, 4096, 128>},
+    {2, 256, 16, 8192, 128, k_count_mp<256, 16, 8192, 128>},
+    {2, 512, 8, 8192, 128, k_count_mp<512, 8, 8192, 128>},
+    {2, 128, 16, 4096, 64, k_count_mp<128, 16, 4096, 64>},
+    {2, 256, 4, 2048, 64, k_count_mp<256, 4, 2048, 64>},
+    {2, 512, 4, 4096, 128, k_count_mp<512, 4, 4096, 128>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 }  // namespace
 
 int num_variants() { return kNumVariants; }
-Variant variant(int i) { return Variant{kVariants[i].bs, kVariants[i].k, kVariants[i].p}; }
+Variant variant(int i) { return Variant{kVariants[i].bs, kVariants[i].k, kVariants[i].p, kVariants[i].algo}; }
 
 size_t scan_tmp_elems(int64_t n_cap) {
   size_t total = 0;

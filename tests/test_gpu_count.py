@@ -14,7 +14,10 @@ from tests import graphs as GR
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(256, 8, 4096), (256, 16, 8192), (512, 8, 8192), (128, 16, 4096), (256, 4, 2048), (512, 4, 4096)]
+# (algo, wg_size, items, ring): algo 0 = streamed kernel, 2 = Merge Path rings only
+VARIANTS = [(0, 256, 8, 4096), (0, 256, 4, 2048), (0, 512, 4, 4096), (0, 128, 16, 4096),
+            (2, 256, 8, 4096), (2, 256, 16, 8192), (2, 512, 8, 8192), (2, 128, 16, 4096), (2, 256, 4, 2048),
+            (2, 512, 4, 4096)]
 
 
 def _check(counter, g, cfg=None):
@@ -41,7 +44,7 @@ def test_random_graphs(counter, g):
 @pytest.mark.parametrize("ranges", [1, 32])
 def test_variants(counter, v, small_max, ranges):
     g = G.rmat(13, 16, seed=21)
-    cfg = dict(wg_size=v[0], items=v[1], ring=v[2], small_max=small_max, ranges_per_wg=ranges)
+    cfg = dict(algo=v[0], wg_size=v[1], items=v[2], ring=v[3], small_max=small_max, ranges_per_wg=ranges)
     _check(counter, g, cfg)
 
 
