@@ -210,14 +210,16 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "cannot write %s\n", fname.c_str());
       return 1;
     }
-    static const int kVar[][4] = {{0, 256, 8, 4096}, {0, 256, 4, 2048}, {0, 512, 4, 4096}, {0, 128, 16, 4096},
-                                  {2, 256, 8, 4096}, {2, 256, 4, 2048}};
+    static const int kVar[][6] = {{0, 256, 8, 4096, 16, 2}, {0, 256, 8, 4096, 16, 4}, {0, 512, 4, 4096, 16, 2},
+                                  {0, 128, 16, 4096, 16, 2}, {2, 256, 8, 4096, 0, 0}};
     for (const auto& v : kVar) {
       tc_cfg cfg{};
       cfg.algo = v[0];
       cfg.wg_size = v[1];
       cfg.items = v[2];
       cfg.ring = v[3];
+      cfg.chunk = v[4];
+      cfg.depth = v[5];
       auto t1 = clk::now();
       rc = tc_count_i32(ctx, nv, nnz, d_off, d_ind, d_out, d_total, &cfg, nullptr);
       int64_t total = 0;
@@ -227,7 +229,8 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "count failed: %s\n", tc_last_error(ctx));
         break;
       }
-      std::fprintf(f, "%d\t%d\t%d\t%d\t%g\t%g\t%g\t%lld\n", v[0], v[1], v[2], v[3], alloc_ms, ex, ex + alloc_ms,
+      std::fprintf(f, "%d\t%d\t%d\t%d\t%d\t%d\t%g\t%g\t%g\t%lld\n", v[0], v[1], v[2], v[3], v[4], v[5], alloc_ms, ex,
+                   ex + alloc_ms,
                    (long long)total);
     }
     std::fclose(f);

@@ -4,6 +4,7 @@
 //   BS  threads per workgroup        CH  elements per streamed chunk (64 B)
 //   P   LDS words = 2P               M   most edges per small tile
 //   K   merge steps per lane in small and ring tiles
+//   SD  streamed chunks in flight per lane
 //
 // Level 1 is the same as k_count_mp: the concatenated intersection work of
 // all edges, [0, W) with |N(u)| + |N(w)| units per edge, is cut into equal
@@ -29,10 +30,17 @@
 //              (equal diagonals per lane, as in k_count_mp)
 //   3. small   whole edges of at most small_max work inside the range,
 //              packed up to M to a tile, Merge Path over the staged lists
-template <int BS, int CH, int P, int M, int K>
+//
+// The phases are compiled into separate kernels (PH bit 1 = phase 1, bit 2
+// = phase 2, bit 4 = phase 3) so the hot wave phase gets the register and
+// LDS budget of its own code; both launches walk the same ranges and split
+// the edges by the same rule (a.wave_acap = the phase-1 A capacity AWT).
+template <int BS, int CH, int P, int M, int K, int SD, int PH, int AWT>
 __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
   constexpr int NW = BS / 64;
-  constexpr int AW = (2 * P) / NW - 1;  // per-wave A capacity
+  constexpr int AW = AWT;                // per-wave A capacity
+  constexpr int BUFW = (PH & 6) ? 2 * P : NW * (AW + 1);
+  static_assert(NW * (AW + 1) <= BUFW, "wave slices must fit the LDS buffer");
   constexpr int ACAP = 2 * P - 1;       // workgroup A capacity
   constexpr int BWCAP = 64 * CH * 64;   // longest B a single wave streams
   constexpr int LANECAP = BS * K;
@@ -43,7 +51,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
   static_assert(CH % 4 == 0, "chunks are read as 16-byte vectors");
   static_assert(LANECAP >= TMAX && LANECAP + 2 * M <= 2 * P && M <= BS, "tile sizes");
 
-  __shared__ int32_t buf[2 * P];
+  __shared__ int32_t buf[BUFW];
   __shared__ int32_t m_ws[M + 1];
   __shared__ int32_t m_la[M], m_lb[M], m_cnt[M], m_u[M], m_w[M];
   __shared__ int64_t m_ag[M], m_bg[M];
@@ -69,7 +77,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
   for (;;) {
     if (tid == 0) s_rng = next_range(a.qctr, nr, blockIdx.x & 7);
     __syncthreads();
-    const int lg = s_rng;
+    const int lg = __builtin_amdgcn_readfirstlane(s_rng);
     __syncthreads();
     if (lg < 0) break;
     int64_t R0, R1;
@@ -101,7 +109,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         const int32_t u = a.eu[e0], w = a.ew[e0];
         const int64_t ab = a.voff[u], bb = a.voff[w];
         const int64_t la = a.voff[u + 1] - ab, lb = a.voff[w + 1] - bb;
-        if ((la < lb ? la : lb) > ACAP) {
+        if ((PH & 2) != 0 && (la < lb ? la : lb) > ACAP) {
           int64_t l2 = t - lb > 0 ? t - lb : 0, h2 = t < la ? t : la;
           while (l2 < h2) {
             const int64_t mid = (l2 + h2) >> 1;
@@ -115,15 +123,17 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
       s_start[1] = (int32_t)(t - i);
     }
     __syncthreads();
-    const int64_t e_first = s_e[0], e_last = s_e[1];
-    const int32_t start_i = s_start[0], start_j = s_start[1];
+    const int64_t e_first = uniform64(s_e[0]), e_last = uniform64(s_e[1]);
+    const int32_t start_i = __builtin_amdgcn_readfirstlane(s_start[0]);
+    const int32_t start_j = __builtin_amdgcn_readfirstlane(s_start[1]);
     __syncthreads();
 
     // ------------------------------------------------------------ phase 1
-    {
-      int32_t* abuf = buf + wid * (AW + 1);
+    if constexpr ((PH & 1) != 0) {
+      const int wv = __builtin_amdgcn_readfirstlane(wid);
+      int32_t* abuf = buf + wv * (AW + 1);
       int32_t cnt_w = 0;
-      for (int64_t e = e_first + wid; e <= e_last; e += NW) {
+      for (int64_t e = e_first + wv; e <= e_last; e += NW) {
         const int64_t we0 = a.wpre[e], L = a.wpre[e + 1] - we0;
         const int64_t t0 = R0 > we0 ? R0 - we0 : 0;
         const int64_t t1 = R1 - we0 < L ? R1 - we0 : L;
@@ -133,7 +143,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         const int32_t la = (int32_t)(a.voff[u + 1] - ab), lb = (int32_t)(a.voff[w + 1] - bb);
         const bool a_is_u = la <= lb;
         const int32_t lA = a_is_u ? la : lb, lB = a_is_u ? lb : la;
-        if (lA > AW || lB > BWCAP) continue;  // phase 2
+        if (lA > a.wave_acap || lB > BWCAP) continue;  // phase 2
         const int64_t gA = a_is_u ? ab : bb, gB = a_is_u ? bb : ab;
         for (int32_t x0 = 0; x0 < lA; x0 += 8 * 64) {
           int32_t v[8];
@@ -157,7 +167,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         if (g0 < g1) {
           const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
           const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
-          c = stream_chunks<CH>(ind4, c_lo, c_hi, g0, g1, abuf, lA);
+          c = stream_chunks<CH, SD>(ind4, c_lo, c_hi, g0, g1, abuf, lA);
         }
         c = wave_sum32(c);
         if (lane == 0 && c) {
@@ -174,6 +184,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
     __syncthreads();
 
     // ------------------------------------------------------------ phase 2
+    if constexpr ((PH & 2) != 0)
     for (int64_t e0 = e_first; e0 <= e_last; e0 += BS) {
       // which of the next BS edges belong to this phase
       bool mine = false;
@@ -186,7 +197,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
           const int32_t u = a.eu[ee], w = a.ew[ee];
           const int64_t la = a.voff[u + 1] - a.voff[u], lb = a.voff[w + 1] - a.voff[w];
           const int64_t lA = la < lb ? la : lb, lB = la < lb ? lb : la;
-          mine = lA > AW || lB > BWCAP;
+          mine = lA > a.wave_acap || lB > BWCAP;
         }
       }
       const uint64_t bal = __ballot(mine);
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
             if (g0 < g1) {
               const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
               const int64_t c_lo = cb + (nc * tid) / BS, c_hi = cb + (nc * (tid + 1)) / BS;
-              cnt = stream_chunks<CH>(ind4, c_lo, c_hi, g0, g1, buf, lA);
+              cnt = stream_chunks<CH, SD>(ind4, c_lo, c_hi, g0, g1, buf, lA);
             }
           } else {
             // both lists longer than the LDS: Merge Path through the rings
@@ -345,6 +356,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
     }
 
     // ------------------------------------------------------------ phase 3
+    if constexpr ((PH & 4) != 0)
     for (int64_t e_scan = e_first; e_scan <= e_last;) {
       const int64_t ee = e_scan + tid;
       bool q = false;

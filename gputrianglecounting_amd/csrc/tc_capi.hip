@@ -47,6 +47,8 @@ tc_cfg builtin_default() {
   c.ring = 4096;
   c.small_max = 2048;
   c.ranges_per_wg = 32;
+  c.chunk = 16;
+  c.depth = 2;
   c.algo = 0;
   return c;
 }
@@ -61,13 +63,16 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->ring) c.ring = in->ring;
     if (in->small_max) c.small_max = in->small_max;
     if (in->ranges_per_wg) c.ranges_per_wg = in->ranges_per_wg;
+    if (in->chunk) c.chunk = in->chunk;
+    if (in->depth) c.depth = in->depth;
     c.algo = in->algo;
   }
   int vi = -1;
   for (int i = 0; i < tcd::num_variants(); ++i) {
     tcd::Variant v = tcd::variant(i);
-    const int want = c.algo == 1 ? 0 : c.algo;  // the simple kernel reuses variant 0's sizes
-    if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want) vi = i;
+    const int want = c.algo == 1 ? 0 : c.algo;  // the simple kernel reuses the streamed sizes
+    const bool stream_ok = want != 0 || (v.chunk == c.chunk && v.depth == c.depth);
+    if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want && stream_ok) vi = i;
   }
   if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 2) return -1;
   const int lanecap = c.wg_size * c.items;
@@ -102,8 +107,8 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
   L.ew = take(4 * (size_t)L.ecap);
   L.ework = take(4 * (size_t)L.ecap);
   L.wpre = take(8 * (size_t)(L.ecap + 1));
-  L.dscal = take(8 * 4);
-  L.qctr = take(4 * 8);
+  L.dscal = take(8 * 8);
+  L.qctr = take(4 * 16);
   const int64_t ncap = std::max<int64_t>(nv + 1, L.ecap);
   L.scan = take(8 * tcd::scan_tmp_elems(ncap));
   L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
@@ -158,9 +163,9 @@ void record(tc_ctx* c, int i, hipStream_t s) {
 
 // Persistent grid for the Merge Path kernel (every workgroup resident) and a
 // fixed grid for the one-lane-per-edge kernel.
-int count_wg(const tc_ctx* c, const tc_cfg& cfg, int vi) {
+int count_wg(const tc_ctx* c, const tc_cfg& cfg, int vi, int which = 0) {
   if (cfg.algo == 1) return 16 * c->cus;
-  int n = tcd::resident_wgs(vi, c->cus);
+  int n = tcd::resident_wgs2(vi, c->cus, which);
   if (n >= 64) n &= ~7;
   return std::max(n, 1);
 }
@@ -183,7 +188,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.qctr = p.qctr;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
-  hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, s);
+  hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, count_wg(c, cfg, vi, 1), s);
   if (e != hipSuccess) return hip_fail(c, e, "count kernel launch");
   return TC_OK;
 }
@@ -531,7 +536,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, size_t bytes) {
   if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  (void)tcd::resident_wgs(vi, c->cus);
+  (void)tcd::resident_wgs2(vi, c->cus, 1);
   return ensure(c, bytes);
 }
 
@@ -556,6 +561,8 @@ std::vector<tc_cfg> tune_candidates() {
         c.small_max = sms[si];
         c.ranges_per_wg = rp;
         c.algo = v.algo;
+        c.chunk = v.chunk;
+        c.depth = v.depth;
         out.push_back(c);
       }
     }

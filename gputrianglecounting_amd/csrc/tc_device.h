@@ -26,11 +26,12 @@ struct CountArgs {
   int32_t max_ranges;           // level-1 ranges: min(W / min_units, max_ranges)
   int32_t min_units;            // smallest level-1 range
   unsigned int* qctr;           // 8 per-XCD range queue counters, zeroed per launch
+  int32_t wave_acap;            // streamed kernel: longest shorter-list a single wave stages
 };
 
 // Kernel variants compiled into the library.
 struct Variant {
-  int wg_size, items, ring, algo;
+  int wg_size, items, ring, algo, chunk, depth;
 };
 int num_variants();
 Variant variant(int i);
@@ -45,8 +46,8 @@ struct Prep {
   int32_t* ew;      // [ecap]
   int32_t* ework;   // [ecap]
   int64_t* wpre;    // [ecap + 1]
-  int64_t* dscal;   // [4]
-  unsigned int* qctr;  // [8]
+  int64_t* dscal;   // [8]: E, W, overflow flag, -, two vertex-chunk counters
+  unsigned int* qctr;  // [16]: two launches' per-XCD range queues
   int64_t* scan_tmp;
   int64_t ecap;
   // oriented copy for the total-only mode
@@ -68,7 +69,9 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
-hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, hipStream_t s);
+// which = 1: the second kernel of a two-launch variant
+int resident_wgs2(int variant_idx, int cus, int which);
+hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, hipStream_t s);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,
                            int cus, hipStream_t s);

@@ -54,6 +54,13 @@ __device__ __forceinline__ int64_t block_sum64(int64_t v, int64_t* red) {
   return s;
 }
 
+// A value every lane of the wave holds, moved to scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Inclusive scan over a wave of 64 lanes.
 __device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
   const int lane = threadIdx.x & 63;
@@ -186,27 +193,52 @@ __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int6
   return mode == 0 ? (wl > vl && dv >= 2 && dw >= 2) : (dv >= 1 && dw >= 1);
 }
 
+// Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
+// so hub rows do not pile up on one wave; each lane keeps 4 arc loads in
+// flight.
+constexpr int kVChunk = 16;
+
+__device__ __forceinline__ int64_t grab_vertices(unsigned long long* ctr) {
+  int64_t v0 = 0;
+  if ((threadIdx.x & 63) == 0) v0 = (int64_t)atomicAdd(ctr, (unsigned long long)kVChunk);
+  return uniform64(__shfl(v0, 0, 64));
+}
+
 __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
                                                     const int32_t* __restrict__ vbase, int64_t nv, int mode,
-                                                    int32_t* __restrict__ ecnt) {
+                                                    int32_t* __restrict__ ecnt, unsigned long long* ctr) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t v = wave; v < nv; v += nwaves) {
-    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-    const int32_t vb = vbase ? vbase[v] : 0;
-    const int32_t vl = (int32_t)(v - vb);
-    int32_t c = 0;
-    if (dv >= (mode == 0 ? 2 : 1)) {
-      for (int64_t k = b + lane; k < e; k += 64) {
-        const int32_t wl = ind[k];
-        const int64_t wg = (int64_t)vb + wl;
-        const int64_t dw = voff[wg + 1] - voff[wg];
-        c += edge_keep(mode, vl, wl, dv, dw);
+  for (;;) {
+    const int64_t v0 = grab_vertices(ctr);
+    if (v0 >= nv) break;
+    const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
+    for (int64_t v = v0; v < v1; ++v) {
+      const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+      const int32_t vb = vbase ? vbase[v] : 0;
+      const int32_t vl = (int32_t)(v - vb);
+      int32_t c = 0;
+      if (dv >= (mode == 0 ? 2 : 1)) {
+        for (int64_t k0 = b; k0 < e; k0 += 256) {
+          int32_t wl[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t k = k0 + lane + 64 * r;
+            wl[r] = k < e ? ind[k] : -1;
+          }
+          int64_t lo[4], hi[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
+            lo[r] = voff[wg];
+            hi[r] = voff[wg + 1];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) c += wl[r] >= 0 && edge_keep(mode, vl, wl[r], dv, hi[r] - lo[r]);
+        }
       }
+      c = wave_sum32(c);
+      if (lane == 0) ecnt[v] = c;
     }
-    c = wave_sum32(c);
-    if (lane == 0) ecnt[v] = c;
   }
 }
 
@@ -214,37 +246,49 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
                                                    const int32_t* __restrict__ vbase, int64_t nv, int mode,
                                                    const int64_t* __restrict__ eoff, int64_t ecap,
                                                    int32_t* __restrict__ eu, int32_t* __restrict__ ew,
-                                                   int32_t* __restrict__ ework) {
+                                                   int32_t* __restrict__ ework, unsigned long long* ctr) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int64_t v = wave; v < nv; v += nwaves) {
-    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-    if (dv < (mode == 0 ? 2 : 1)) continue;
-    const int32_t vb = vbase ? vbase[v] : 0;
-    const int32_t vl = (int32_t)(v - vb);
-    int64_t pos = eoff[v];
-    for (int64_t k0 = b; k0 < e; k0 += 64) {
-      const int64_t k = k0 + lane;
-      bool keep = false;
-      int64_t wg = 0, dw = 0;
-      if (k < e) {
-        const int32_t wl = ind[k];
-        wg = (int64_t)vb + wl;
-        dw = voff[wg + 1] - voff[wg];
-        keep = edge_keep(mode, vl, wl, dv, dw);
-      }
-      const uint64_t m = __ballot(keep);
-      if (keep) {
-        const int64_t idx = pos + __popcll(m & lt);
-        if (idx < ecap) {
-          eu[idx] = (int32_t)v;
-          ew[idx] = (int32_t)wg;
-          ework[idx] = (int32_t)(dv + dw);
+  for (;;) {
+    const int64_t v0 = grab_vertices(ctr);
+    if (v0 >= nv) break;
+    const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
+    for (int64_t v = v0; v < v1; ++v) {
+      const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+      if (dv < (mode == 0 ? 2 : 1)) continue;
+      const int32_t vb = vbase ? vbase[v] : 0;
+      const int32_t vl = (int32_t)(v - vb);
+      int64_t pos = eoff[v];
+      for (int64_t k0 = b; k0 < e; k0 += 256) {
+        int32_t wl[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t k = k0 + lane + 64 * r;
+          wl[r] = k < e ? ind[k] : -1;
+        }
+        int64_t lo[4], hi[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
+          lo[r] = voff[wg];
+          hi[r] = voff[wg + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t dw = hi[r] - lo[r];
+          const bool keep = wl[r] >= 0 && edge_keep(mode, vl, wl[r], dv, dw);
+          const uint64_t m = __ballot(keep);
+          if (keep) {
+            const int64_t idx = pos + __popcll(m & lt);
+            if (idx < ecap) {
+              eu[idx] = (int32_t)v;
+              ew[idx] = (int32_t)((int64_t)vb + wl[r]);
+              ework[idx] = (int32_t)(dv + dw);
+            }
+          }
+          pos += __popcll(m);
         }
       }
-      pos += __popcll(m);
     }
   }
 }
@@ -343,12 +387,21 @@ __device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
 // Count the elements of chunks [c_lo, c_hi) of a 16-byte aligned array base
 // (chunk c = elements [c*CH, c*CH + CH)), restricted to element positions
 // [g0, g1), that occur in the sorted list A[0, lA) held in LDS with
-// A[lA] = INT_MAX.  Chunk c+1 is loaded while chunk c is compared.
-template <int CH>
+// A[lA] = INT_MAX.  D chunks are in flight per lane: chunk c + D is loaded
+// into the register slot chunk c is compared from.
+template <int CH, int D>
 __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
                                                  int64_t g0, int64_t g1, const int32_t* A, int32_t lA) {
   constexpr int Q = CH / 4;
   if (c_lo >= c_hi) return 0;
+  const int4* b4 = reinterpret_cast<const int4*>(base);
+  int4 slot[D][Q];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (c_lo + k < c_hi) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) slot[k][q] = b4[(c_lo + k) * Q + q];
+    }
   const int64_t gf = c_lo * CH > g0 ? c_lo * CH : g0;
   const int32_t bf = base[gf];
   int32_t lo = 0, hi = lA;  // lower bound of bf in A
@@ -360,41 +413,38 @@ __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ bas
   int32_t pos = lo;
   int32_t acur = A[pos];
   int32_t cnt = 0;
-  const int4* b4 = reinterpret_cast<const int4*>(base);
-  int4 cur[Q];
+  for (int64_t c0 = c_lo; c0 < c_hi; c0 += D) {
 #pragma unroll
-  for (int q = 0; q < Q; ++q) cur[q] = b4[c_lo * Q + q];
-  for (int64_t c = c_lo; c < c_hi; ++c) {
-    int4 nxt[Q];
-    if (c + 1 < c_hi) {
+    for (int k = 0; k < D; ++k) {
+      const int64_t c = c0 + k;
+      if (c < c_hi) {
+        int32_t b[CH];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) nxt[q] = b4[(c + 1) * Q + q];
-    }
-    int32_t b[CH];
+        for (int q = 0; q < Q; ++q) {
+          b[4 * q] = slot[k][q].x;
+          b[4 * q + 1] = slot[k][q].y;
+          b[4 * q + 2] = slot[k][q].z;
+          b[4 * q + 3] = slot[k][q].w;
+        }
+        if (c + D < c_hi) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      b[4 * q] = cur[q].x;
-      b[4 * q + 1] = cur[q].y;
-      b[4 * q + 2] = cur[q].z;
-      b[4 * q + 3] = cur[q].w;
-    }
-    const int64_t cbase = c * CH;
-    if (cbase < g0 || cbase + CH > g1) {
+          for (int q = 0; q < Q; ++q) slot[k][q] = b4[(c + D) * Q + q];
+        }
+        const int64_t cbase = c * CH;
+        if (cbase < g0 || cbase + CH > g1) {
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
-        const int64_t g = cbase + k;
-        if (g < g0 || g >= g1) b[k] = -1;  // ids are >= 0: never matches, never advances
+          for (int j = 0; j < CH; ++j) {
+            const int64_t g = cbase + j;
+            if (g < g0 || g >= g1) b[j] = -1;  // ids are >= 0: never matches, never advances
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int32_t bv = b[j];
+          while (acur < bv) acur = A[++pos];
+          cnt += (acur == bv);
+        }
       }
-    }
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      const int32_t bv = b[k];
-      while (acur < bv) acur = A[++pos];
-      cnt += (acur == bv);
-    }
-    if (c + 1 < c_hi) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) cur[q] = nxt[q];
     }
   }
   return cnt;
@@ -520,30 +570,41 @@ unsigned grid_for(int64_t n, int bs, int64_t cap = 65536) {
   return (unsigned)g;
 }
 
-// Compiled count kernels: algo 0 = streamed (k_count_bs), 2 = Merge Path
-// rings only (k_count_mp).  {algo, BS, K, P, M}.
+// Compiled count kernels: algo 0 = streamed (k_count_bs, launched as a wave
+// kernel fn then a workgroup/small-tile kernel fn2), 2 = Merge Path rings
+// only (k_count_mp).  {algo, BS, K, P, M, chunk, depth, wave A cap}.
 struct VariantDef {
-  int algo, bs, k, p, m;
+  int algo, bs, k, p, m, ch, sd, awt;
   void (*fn)(CountArgs);
+  void (*fn2)(CountArgs);
 };
+#define TC_BS(BS, CH, P, M, K, SD, AWT)                                                        \
+  {0, BS, K, P, M, CH, SD, AWT, k_count_bs<BS, CH, P, M, K, SD, 1, AWT>,                    \
+   k_count_bs<BS, CH, P, M, K, SD, 6, AWT>}
+#define TC_MP(BS, K, P, M) {2, BS, K, P, M, 0, 0, 0, k_count_mp<BS, K, P, M>, nullptr}
 const VariantDef kVariants[] = {
-    {0, 256, 8, 4096, 128, k_count_bs<256, 16, 4096, 128, 8>},
-    {0, 256, 4, 2048, 64, k_count_bs<256, 16, 2048, 64, 4>},
-    {0, 512, 4, 4096, 128, k_count_bs<512, 16, 4096, 128, 4>},
-    {0, 128, 16, 4096, 64, k_count_bs<128, 16, 4096, 64, 16>},
-    {2, 256, 8, 4096, 128, k_count_mp<256, 8, 4096, 128>},
-    {2, 256, 16, 8192, 128, k_count_mp<256, 16, 8192, 128>},
-    {2, 512, 8, 8192, 128, k_count_mp<512, 8, 8192, 128>},
-    {2, 128, 16, 4096, 64, k_count_mp<128, 16, 4096, 64>},
-    {2, 256, 4, 2048, 64, k_count_mp<256, 4, 2048, 64>},
-    {2, 512, 4, 4096, 128, k_count_mp<512, 4, 4096, 128>},
+    TC_BS(256, 16, 4096, 128, 8, 2, 2039),
+    TC_BS(256, 16, 4096, 128, 8, 4, 2039),
+    TC_BS(512, 16, 4096, 128, 4, 2, 1019),
+    TC_BS(128, 16, 4096, 64, 16, 2, 4087),
+    TC_MP(256, 8, 4096, 128),
+    TC_MP(256, 16, 8192, 128),
+    TC_MP(512, 8, 8192, 128),
+    TC_MP(128, 16, 4096, 64),
+    TC_MP(256, 4, 2048, 64),
+    TC_MP(512, 4, 4096, 128),
 };
+#undef TC_BS
+#undef TC_MP
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 }  // namespace
 
 int num_variants() { return kNumVariants; }
-Variant variant(int i) { return Variant{kVariants[i].bs, kVariants[i].k, kVariants[i].p, kVariants[i].algo}; }
+Variant variant(int i) {
+  const VariantDef& d = kVariants[i];
+  return Variant{d.bs, d.k, d.p, d.algo, d.ch, d.sd};
+}
 
 size_t scan_tmp_elems(int64_t n_cap) {
   size_t total = 0;
@@ -581,14 +642,18 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode, int cus,
                               hipStream_t s) {
-  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 32);
+  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 16);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
+  hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
+  if (e != hipSuccess) return e;
   if (nv > 0)
-    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode, p.ecnt);
-  hipError_t e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.eoff, p.scan_tmp, s);
+    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode, p.ecnt,
+                       ctr);
+  e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.eoff, p.scan_tmp, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
-                       (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework);
+                       (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework, ctr + 1);
   hipLaunchKernelGGL(k_edge_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.eoff, nv, p.ecap, p.dscal);
   e = scan_excl<int32_t>(p.ework, p.dscal, p.ecap, p.wpre, p.scan_tmp, s);
   if (e != hipSuccess) return e;
@@ -607,28 +672,43 @@ hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus,
   return hipGetLastError();
 }
 
-int resident_wgs(int vi, int cus) {
-  static int occ[kNumVariants] = {0};
-  if (vi < 0 || vi >= kNumVariants) return 0;
-  if (!occ[vi]) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(kVariants[vi].fn),
-                                                     kVariants[vi].bs, 0) != hipSuccess || n < 1)
-      n = 1;
-    occ[vi] = n;
-  }
-  return occ[vi] * cus;
+int occupancy(void (*fn)(CountArgs), int bs) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(fn), bs, 0) != hipSuccess || n < 1)
+    n = 1;
+  return n;
 }
 
-hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, hipStream_t s) {
+// Resident workgroups of a variant's first (or second) kernel.
+int resident_wgs2(int vi, int cus, int which) {
+  static int occ[kNumVariants][2] = {};
+  if (vi < 0 || vi >= kNumVariants) return 0;
+  if (!occ[vi][0]) {
+    occ[vi][0] = occupancy(kVariants[vi].fn, kVariants[vi].bs);
+    occ[vi][1] = kVariants[vi].fn2 ? occupancy(kVariants[vi].fn2, kVariants[vi].bs) : occ[vi][0];
+  }
+  return occ[vi][which] * cus;
+}
+
+int resident_wgs(int vi, int cus) { return resident_wgs2(vi, cus, 0); }
+
+hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, int nwg2, hipStream_t s) {
   if (algo == 1) {
     hipLaunchKernelGGL(k_count_simple, dim3(nwg), dim3(256), 0, s, a);
     return hipGetLastError();
   }
   if (vi < 0 || vi >= kNumVariants) return hipErrorInvalidValue;
-  hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.qctr), 8, 0, s);
+  const VariantDef& v = kVariants[vi];
+  hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.qctr), 16, 0, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kVariants[vi].fn, dim3(nwg), dim3(kVariants[vi].bs), 0, s, a);
+  CountArgs b = a;
+  b.wave_acap = v.awt;
+  hipLaunchKernelGGL(v.fn, dim3(nwg), dim3(v.bs), 0, s, b);
+  if (v.fn2) {
+    // same ranges (a.max_ranges), its own range queues
+    b.qctr = a.qctr + 8;
+    hipLaunchKernelGGL(v.fn2, dim3(nwg2 > 0 ? nwg2 : 1), dim3(v.bs), 0, s, b);
+  }
   return hipGetLastError();
 }
 

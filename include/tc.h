@@ -46,8 +46,11 @@ typedef struct tc_ctx tc_ctx;
  *   ranges_per_wg  level-1 split: the edge-work range [0, W) is cut into
  *                  ranges_per_wg * (resident workgroups) equal ranges that a
  *                  persistent grid pulls from per-XCD queues
- *   algo           0 = two-level Merge Path (default), 1 = one lane per
- *                  edge (simple, for cross-checks) */
+ *   algo           0 = streamed two-level kernel (default), 1 = one lane
+ *                  per edge (simple, for cross-checks), 2 = Merge Path
+ *                  through LDS rings for every long edge
+ *   chunk          algo 0: elements per streamed chunk (16 or 32)
+ *   depth          algo 0: streamed chunks in flight per lane (2 or 4) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
@@ -55,6 +58,8 @@ typedef struct tc_cfg {
   int32_t small_max;
   int32_t ranges_per_wg;
   int32_t algo;
+  int32_t chunk;
+  int32_t depth;
 } tc_cfg;
 
 enum {

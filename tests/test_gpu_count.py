@@ -15,9 +15,9 @@ from tests import graphs as GR
 pytestmark = pytest.mark.gpu
 
 # (algo, wg_size, items, ring): algo 0 = streamed kernel, 2 = Merge Path rings only
-VARIANTS = [(0, 256, 8, 4096), (0, 256, 4, 2048), (0, 512, 4, 4096), (0, 128, 16, 4096),
-            (2, 256, 8, 4096), (2, 256, 16, 8192), (2, 512, 8, 8192), (2, 128, 16, 4096), (2, 256, 4, 2048),
-            (2, 512, 4, 4096)]
+VARIANTS = [(0, 256, 8, 4096, 16, 2), (0, 256, 8, 4096, 16, 4), (0, 512, 4, 4096, 16, 2), (0, 128, 16, 4096, 16, 2),
+            (2, 256, 8, 4096, 0, 0), (2, 256, 16, 8192, 0, 0), (2, 512, 8, 8192, 0, 0), (2, 128, 16, 4096, 0, 0),
+            (2, 256, 4, 2048, 0, 0), (2, 512, 4, 4096, 0, 0)]
 
 
 def _check(counter, g, cfg=None):
@@ -44,7 +44,8 @@ def test_random_graphs(counter, g):
 @pytest.mark.parametrize("ranges", [1, 32])
 def test_variants(counter, v, small_max, ranges):
     g = G.rmat(13, 16, seed=21)
-    cfg = dict(algo=v[0], wg_size=v[1], items=v[2], ring=v[3], small_max=small_max, ranges_per_wg=ranges)
+    cfg = dict(algo=v[0], wg_size=v[1], items=v[2], ring=v[3], chunk=v[4], depth=v[5], small_max=small_max,
+               ranges_per_wg=ranges)
     _check(counter, g, cfg)
 
 
