@@ -152,7 +152,9 @@ def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
 
 
 def traffic_from_profile(workload: str):
-    path = os.path.join(ROOT, "profiles", f"count_mp_{workload}_hbm.json")
+    """HBM bytes per count step (both count kernels) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"count_{workload}_hbm.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -165,7 +167,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er"])
+    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er", "batch"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=8,ring=4096")
@@ -179,6 +181,18 @@ def main() -> int:
     import torch
 
     from gputrianglecounting_amd.tc import TriangleCounter
+
+    if args.workload == "batch":
+        tc = TriangleCounter(0)
+        res = run_batch(tc, steps=max(2, args.steps), warmup=args.warmup)
+        value = res["batched_graphs_per_s"]
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "graphs/s", "n_gpus": 1, "steps": max(2, args.steps),
+                          "warmup": args.warmup, "ms_per_step": res["batched_ms"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+                          "data": "synthetic: 4096 seeded ER/R-MAT/RGG graphs",
+                          "config": {"workload": "batch_4096_mixed"}, "batched": res}), flush=True)
+        tc.close()
+        return 0
 
     log(f"generating {args.workload}")
     t0 = time.time()

@@ -1,0 +1,192 @@
+// Device-memory admission for the serving loop: a first-fit arena over a
+// fixed byte budget, a FIFO of waiting requests, and the layout of one batch
+// block.  Header-only and free of HIP so the policy can be exercised on the
+// CPU with a fake budget.
+//
+// A batch of G requests occupies ONE arena block laid out as the batched
+// counter's packed CSR (tab rows {v_base, arc_base}, concatenated offsets
+// and ids, then the per-vertex counts and per-graph totals), so the block
+// can be handed to the batched entry point as it is.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <iterator>
+#include <map>
+#include <vector>
+
+namespace tcs {
+
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_round(size_t x) { return (x + kArenaAlign - 1) & ~(kArenaAlign - 1); }
+
+// First-fit allocator over [0, capacity): free blocks kept by offset and
+// coalesced on release.  Offsets are multiples of kArenaAlign.
+class Arena {
+ public:
+  explicit Arena(size_t capacity) : cap_(capacity - capacity % kArenaAlign) {
+    if (cap_) free_[0] = cap_;
+  }
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  size_t largest_free() const {
+    size_t m = 0;
+    for (const auto& f : free_) m = f.second > m ? f.second : m;
+    return m;
+  }
+
+  // Offset of a block of at least `bytes`, or SIZE_MAX when nothing fits.
+  size_t alloc(size_t bytes) {
+    bytes = arena_round(bytes ? bytes : 1);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > bytes) free_[off + bytes] = len - bytes;
+      used_ += bytes;
+      if (used_ > peak_) peak_ = used_;
+      return off;
+    }
+    return SIZE_MAX;
+  }
+
+  void release(size_t off, size_t bytes) {
+    bytes = arena_round(bytes ? bytes : 1);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  size_t cap_;
+  size_t used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+};
+
+// Byte offsets of the regions of one batch block, each kArenaAlign aligned:
+//   tab   int64[2(G+1)]     rows {v_base, arc_base}, row G = totals
+//   off   int32[nv + G]     graph g's offsets at v_base[g] + g
+//   ind   int32[nnz]        graph g's ids at arc_base[g]
+//   out   int32[max(nv,1)]  per-vertex counts
+//   tot   int64[G]          per-graph totals
+// [0, upload_end) goes host -> device, [out, bytes) device -> host.
+struct BatchLayout {
+  size_t tab, off, ind, out, tot, upload_end, bytes;
+};
+inline BatchLayout batch_layout(int64_t G, int64_t nv, int64_t nnz) {
+  BatchLayout L;
+  size_t p = 0;
+  L.tab = p;
+  p += arena_round(16 * (size_t)(G + 1));
+  L.off = p;
+  p += arena_round(4 * (size_t)(nv + G));
+  L.ind = p;
+  L.upload_end = p + 4 * (size_t)nnz;
+  p += arena_round(4 * (size_t)nnz);
+  L.out = p;
+  p += arena_round(4 * (size_t)(nv > 0 ? nv : 1));
+  L.tot = p;
+  p += arena_round(8 * (size_t)G);
+  L.bytes = p;
+  return L;
+}
+
+// Writes graphs 0..G-1 (nv[g] vertices, nnz[g] arcs, offsets off[g][0..nv],
+// ids ind[g][0..nnz)) into the [0, upload_end) part of a host block laid
+// out by batch_layout(G, sum nv, sum nnz); returns that layout.
+inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
+                              const int32_t* const* ind, char* blk) {
+  int64_t v = 0, a = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    v += nv[g];
+    a += nnz[g];
+  }
+  const BatchLayout L = batch_layout(G, v, a);
+  int64_t* tab = reinterpret_cast<int64_t*>(blk + L.tab);
+  int32_t* offc = reinterpret_cast<int32_t*>(blk + L.off);
+  int32_t* indc = reinterpret_cast<int32_t*>(blk + L.ind);
+  v = a = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    tab[2 * g] = v;
+    tab[2 * g + 1] = a;
+    std::memcpy(offc + v + g, off[g], 4 * ((size_t)nv[g] + 1));
+    if (nnz[g]) std::memcpy(indc + a, ind[g], 4 * (size_t)nnz[g]);
+    v += nv[g];
+    a += nnz[g];
+  }
+  tab[2 * G] = v;
+  tab[2 * G + 1] = a;
+  return L;
+}
+
+// Batch block size as a sum over requests: each request contributes its
+// exact share (a tab row, nv+1 offsets, nnz ids, nv counts, one total) and
+// the block adds kBatchFixed, which covers the tab's closing row and the
+// rounding of the five regions (each < kArenaAlign).  So
+// batch_layout(G, sum nv, sum nnz).bytes <= batch_bytes(sum of shares).
+constexpr size_t kBatchFixed = 6 * kArenaAlign;
+inline size_t request_share(int64_t nv, int64_t nnz) { return 28 + 8 * (size_t)nv + 4 * (size_t)nnz; }
+inline size_t batch_bytes(size_t share_sum) { return arena_round(kBatchFixed + share_sum); }
+// Arena bytes of a request batched alone.
+inline size_t request_bytes(int64_t nv, int64_t nnz) { return batch_bytes(request_share(nv, nnz)); }
+
+// FIFO admission of batches: the next batch is the longest prefix of the
+// queue (at most max_n requests) whose block fits the largest free arena
+// block.  Later requests never overtake the head, so a large request cannot
+// starve; when the head alone does not fit nothing is admitted.
+struct Pending {
+  int64_t id;
+  size_t share;
+};
+
+class Admission {
+ public:
+  explicit Admission(size_t budget) : arena_(budget) {}
+  Arena& arena() { return arena_; }
+  // False when the request can never fit (larger than the whole budget).
+  bool fits_ever(size_t share) const { return batch_bytes(share) <= arena_.capacity(); }
+  void push(int64_t id, size_t share) { q_.push_back(Pending{id, share}); }
+  size_t waiting() const { return q_.size(); }
+  // Admit one batch: fills ids and returns the block offset and size, or
+  // SIZE_MAX (ids empty) when the head does not fit or nothing waits.
+  size_t admit_batch(int max_n, std::vector<int64_t>& ids, size_t& bytes) {
+    ids.clear();
+    bytes = 0;
+    const size_t room = arena_.largest_free();
+    size_t sum = 0;
+    int n = 0;
+    for (const Pending& p : q_) {
+      if (n >= max_n || batch_bytes(sum + p.share) > room) break;
+      sum += p.share;
+      ++n;
+    }
+    if (n == 0) return SIZE_MAX;
+    bytes = batch_bytes(sum);
+    const size_t off = arena_.alloc(bytes);
+    if (off == SIZE_MAX) return SIZE_MAX;  // not reached: room >= bytes
+    for (int i = 0; i < n; ++i) {
+      ids.push_back(q_.front().id);
+      q_.pop_front();
+    }
+    return off;
+  }
+
+ private:
+  Arena arena_;
+  std::deque<Pending> q_;
+};
+
+}  // namespace tcs

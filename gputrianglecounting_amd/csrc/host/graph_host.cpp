@@ -679,3 +679,79 @@ tch_graph* tch_batch_graph(const tch_batch* b, int32_t g) {
 void tch_batch_free(tch_batch* b) { delete b; }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ admission sim
+#include "admission.h"
+
+extern "C" {
+
+int64_t tch_request_bytes(int64_t nv, int64_t nnz) {
+  if (nv < 0 || nnz < 0) return fail(TCH_EINVAL, "invalid argument");
+  return (int64_t)tcs::request_bytes(nv, nnz);
+}
+
+int64_t tch_batch_block_bytes(int32_t G, int64_t nv_total, int64_t nnz_total) {
+  if (G < 1 || nv_total < 0 || nnz_total < 0) return fail(TCH_EINVAL, "invalid argument");
+  return (int64_t)tcs::batch_layout(G, nv_total, nnz_total).bytes;
+}
+
+int64_t tch_pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
+                       const int32_t* const* ind, void* block, int64_t block_bytes) {
+  if (G < 1 || !nv || !nnz || !off || !ind || !block) return fail(TCH_EINVAL, "invalid argument");
+  int64_t v = 0, a = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    if (nv[g] < 0 || nnz[g] < 0 || !off[g] || (nnz[g] > 0 && !ind[g])) return fail(TCH_EINVAL, "invalid graph");
+    v += nv[g];
+    a += nnz[g];
+  }
+  if ((int64_t)tcs::batch_layout(G, v, a).upload_end > block_bytes) return fail(TCH_EINVAL, "block too small");
+  return (int64_t)tcs::pack_block(G, nv, nnz, off, ind, static_cast<char*>(block)).upload_end;
+}
+
+int tch_admission_sim(int64_t budget, int32_t max_batch, int32_t nsteps, const int64_t* script,
+                      int32_t* admitted_at, int32_t* batch_of, int64_t* peak_used) {
+  if (budget <= 0 || max_batch < 1 || nsteps < 0 || (nsteps > 0 && (!script || !admitted_at || !batch_of)))
+    return fail(TCH_EINVAL, "invalid argument");
+  tcs::Admission adm((size_t)budget);
+  struct Blk {
+    size_t off, bytes;
+    int32_t live;
+  };
+  std::vector<Blk> blks;
+  std::vector<int32_t> state;  // per request: 0 waiting, 1 admitted, 2 completed
+  std::vector<int64_t> ids;
+  int admitted = 0;
+  for (int32_t i = 0; i < nsteps; ++i) {
+    const int64_t x = script[i];
+    if (x > 0) {
+      if (!adm.fits_ever((size_t)x)) return fail(TCH_ENOMEM, "request larger than the budget");
+      const int32_t r = (int32_t)state.size();
+      adm.push(r, (size_t)x);
+      state.push_back(0);
+      admitted_at[r] = -1;
+      batch_of[r] = -1;
+    } else if (x < 0) {
+      const int64_t r = -x - 1;
+      if (r >= (int64_t)state.size() || state[r] != 1) return fail(TCH_EINVAL, "completion of a request not in flight");
+      state[r] = 2;
+      Blk& b = blks[batch_of[r]];
+      if (--b.live == 0) adm.arena().release(b.off, b.bytes);
+    }
+    for (;;) {
+      size_t bytes = 0;
+      const size_t off = adm.admit_batch(max_batch, ids, bytes);
+      if (off == SIZE_MAX) break;
+      for (int64_t r : ids) {
+        state[r] = 1;
+        admitted_at[r] = i;
+        batch_of[r] = (int32_t)blks.size();
+      }
+      blks.push_back(Blk{off, bytes, (int32_t)ids.size()});
+      admitted += (int)ids.size();
+    }
+  }
+  if (peak_used) *peak_used = (int64_t)adm.arena().peak();
+  return admitted;
+}
+
+}  // extern "C"

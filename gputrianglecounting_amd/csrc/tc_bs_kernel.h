@@ -67,10 +67,13 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int64_t E = a.dscal[0], W = a.dscal[1];
   const int nr = level1_parts(W, a.max_ranges, a.min_units);
-  // Chunks are 16-byte aligned in memory: element g of ind is element
-  // g + sh of the aligned base ind4 (sh < 4 words of the 16-byte block that
-  // holds ind[0], so no access leaves the pages ind occupies).
-  const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(a.ind) >> 2) & 3);
+  // Chunks are CH*4-byte aligned in memory: element g of ind is element
+  // g + sh of the aligned base ind4 (sh < CH words of the aligned block that
+  // holds ind[0]).  An aligned chunk never crosses a page boundary, and every
+  // chunk read holds at least one element of ind, so every read stays on a
+  // page that ind occupies, whatever ind's own alignment and length.
+  static_assert((CH & (CH - 1)) == 0, "chunk length must be a power of two");
+  const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(a.ind) >> 2) & (CH - 1));
   const int32_t* ind4 = a.ind - sh;
   int64_t my_sum = 0;
   int par = 1;

@@ -18,6 +18,7 @@
 // count added to both endpoints, and the work split is by cumulative
 // intersection length, so a hub's edges spread over many workgroups.
 #include <climits>
+#include <mutex>
 
 #include "tc_device.h"
 
@@ -88,7 +89,9 @@ __global__ void k_voff_from_off32(const int32_t* __restrict__ off, int64_t n1, i
 }
 
 // Batch: global vertex v of graph g (v_base[g] <= v < v_base[g+1]) gets
-// voff[v] = arc_base[g] + off_cat[v_base[g] + g + (v - v_base[g])].
+// voff[v] = arc_base[g] + off_cat[v_base[g] + g + (v - v_base[g])].  The
+// graphs' arcs are contiguous (arc_base[g+1] = arc_base[g] + nnz_g), so
+// voff[v+1] ends every row, including the last row of each graph.
 __global__ void k_voff_batch(const int64_t* __restrict__ tab, int32_t G, const int32_t* __restrict__ off_cat,
                              int64_t nv, int64_t* __restrict__ voff, int32_t* __restrict__ vbase) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v <= nv; v += (int64_t)gridDim.x * blockDim.x) {
@@ -384,8 +387,8 @@ __device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
 
 #include "tc_mp_kernel.h"
 
-// Count the elements of chunks [c_lo, c_hi) of a 16-byte aligned array base
-// (chunk c = elements [c*CH, c*CH + CH)), restricted to element positions
+// Count the elements of chunks [c_lo, c_hi) of a CH*4-byte aligned array
+// base (chunk c = elements [c*CH, c*CH + CH)), restricted to element positions
 // [g0, g1), that occur in the sorted list A[0, lA) held in LDS with
 // A[lA] = INT_MAX.  D chunks are in flight per lane: chunk c + D is loaded
 // into the register slot chunk c is compared from.
@@ -680,12 +683,16 @@ int occupancy(void (*fn)(CountArgs), int bs) {
 }
 
 // Resident workgroups of a variant's first (or second) kernel.
+// Thread-safe: the per-variant figures are computed once under a lock.
 int resident_wgs2(int vi, int cus, int which) {
+  static std::mutex mu;
   static int occ[kNumVariants][2] = {};
-  if (vi < 0 || vi >= kNumVariants) return 0;
+  if (vi < 0 || vi >= kNumVariants || which < 0 || which > 1) return 0;
+  std::lock_guard<std::mutex> lk(mu);
   if (!occ[vi][0]) {
+    occ[vi][1] = kVariants[vi].fn2 ? occupancy(kVariants[vi].fn2, kVariants[vi].bs) : 0;
     occ[vi][0] = occupancy(kVariants[vi].fn, kVariants[vi].bs);
-    occ[vi][1] = kVariants[vi].fn2 ? occupancy(kVariants[vi].fn2, kVariants[vi].bs) : occ[vi][0];
+    if (!occ[vi][1]) occ[vi][1] = occ[vi][0];
   }
   return occ[vi][which] * cus;
 }

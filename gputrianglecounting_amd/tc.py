@@ -132,7 +132,8 @@ EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes
            "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
-           "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32"]
+           "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
+           "tc_server_submit", "tc_server_wait", "tc_server_get_stats"]
 
 
 def _torch():

@@ -200,6 +200,47 @@ int tc_last_edge_work(tc_ctx* ctx, int64_t out[2]);
  * against a transfer of known size. */
 int tc_copy_i32(tc_ctx* ctx, int32_t* d_dst, const int32_t* d_src, int64_t n, void* stream);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop that counts submitted graphs in batches as they arrive.  A
+ * worker thread forms batches first in first out: the next batch is the
+ * longest prefix of the queue (at most max_batch_graphs requests) whose
+ * block fits the largest free block of a device arena of hbm_budget bytes.
+ * The block holds the batch in tc_batch_count_i32's packed layout (table,
+ * offsets, ids) followed by its counts and totals, and stays allocated
+ * until the batch completes, so hbm_budget bounds the device memory of the
+ * requests in flight; each slot's context holds its own scratch on top.
+ * A batch is packed into a pinned host mirror of its block, uploaded with
+ * one copy, counted and copied back on one of `slots` launch slots (each
+ * with its own stream and context), so batches overlap on the device while
+ * later requests wait for memory.  Counts are those of tc_count_i32 on
+ * each graph alone.  No reference counterpart (the reference counts one
+ * graph per process, gpurun.cu:2-56). */
+typedef struct tc_server tc_server;
+typedef struct tc_server_cfg {
+  int32_t device;
+  int32_t slots;            /* batches in flight, 1..8 */
+  int32_t max_batch_graphs; /* most graphs per launch */
+  int32_t reserved;
+  int64_t hbm_budget;       /* bytes of the arena holding the batches in flight */
+} tc_server_cfg;
+typedef struct tc_server_stats {
+  int64_t submitted, completed, rejected, batches;
+  int64_t bytes_in_use, peak_bytes, waiting, in_flight;
+} tc_server_stats;
+int tc_server_create(const tc_server_cfg* cfg, tc_server** out);
+/* Finishes all submitted work, then frees everything. */
+int tc_server_destroy(tc_server* s);
+/* Copies the host CSR and queues it; returns a request id >= 0, or a
+ * negative TC_E* code (TC_ENOMEM if the request can never fit the budget). */
+int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h_off, const int32_t* h_ind);
+/* Waits up to timeout_ms (< 0: no limit) for request id; on completion
+ * copies its nv counts to h_out (nullable), the total to h_total
+ * (nullable) and the submit-to-completion time to latency_ms (nullable),
+ * and forgets the request.  Returns TC_OK, 1 on timeout, or a TC_E* code. */
+int tc_server_wait(tc_server* s, int64_t id, int32_t* h_out, int64_t* h_total, int32_t timeout_ms,
+                   double* latency_ms);
+int tc_server_get_stats(tc_server* s, tc_server_stats* st);
+
 /* Number of compute units of the context's device (for sizing). */
 int tc_device_cus(const tc_ctx* ctx);
 

@@ -112,6 +112,32 @@ const int32_t* tch_batch_kind(const tch_batch* b);
 tch_graph*     tch_batch_graph(const tch_batch* b, int32_t g);
 void           tch_batch_free(tch_batch* b);
 
+/* ------------------------------------------------------- serving admission
+ * The serving loop's admission policy run without a device.  Requests wait
+ * first in first out; the next batch is the longest prefix of the queue (at
+ * most max_batch requests) whose block fits the largest free block of a
+ * first-fit arena of `budget` bytes; the head is never overtaken.
+ * script[i] > 0 submits a request whose share of a batch block is that many
+ * bytes (tch_request_bytes minus the fixed part: see tcs::request_share);
+ * script[i] < 0 completes request -script[i]-1, and a batch's block is
+ * released when all its requests have completed.  After every step batches
+ * are admitted while the head fits.  admitted_at[r] gets the step at which
+ * request r was admitted (-1 if never) and batch_of[r] its batch number;
+ * returns the number of requests admitted, or a negative TCH_E* code
+ * (TCH_ENOMEM for a request that can never fit). */
+int tch_admission_sim(int64_t budget, int32_t max_batch, int32_t nsteps, const int64_t* script,
+                      int32_t* admitted_at, int32_t* batch_of, int64_t* peak_used);
+/* Arena bytes of a request of nv vertices and nnz arcs batched alone. */
+int64_t tch_request_bytes(int64_t nv, int64_t nnz);
+/* Exact bytes of the block of a batch of G graphs with these totals. */
+int64_t tch_batch_block_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
+/* Packs G graphs into a host block as the serving loop uploads it: the
+ * int64 table at byte 0, offsets and ids at the regions tcs::batch_layout
+ * gives (each 256-byte aligned).  Returns the bytes written (the upload
+ * size) or a negative TCH_E* code. */
+int64_t tch_pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
+                       const int32_t* const* ind, void* block, int64_t block_bytes);
+
 enum {
   TCH_OK = 0,
   TCH_EINVAL = -1,     /* bad argument */

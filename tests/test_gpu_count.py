@@ -62,6 +62,24 @@ def test_repeat_deterministic(counter):
     assert ta == tb
 
 
+def test_ind_at_every_alignment(counter):
+    """ind handed over as a view at every word offset of a 64-byte block and
+    ending flush with its allocation: the streamed loads read aligned 64-byte
+    chunks around the lists, which must give the same counts at every shift."""
+    import torch
+
+    g = G.rmat(13, 24, seed=9)
+    ref, rtot = oracle.per_vertex(g.off, g.ind)
+    off = torch.from_numpy(g.off).to("cuda")
+    for k in range(16):
+        print(f"case ind offset {k} words", flush=True)
+        buf = torch.full((g.nnz + k,), -1, dtype=torch.int32, device="cuda")
+        buf[k:] = torch.from_numpy(g.ind).to("cuda")
+        out, tot = counter.count(off, buf[k:])
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+        assert int(tot.item()) == rtot
+
+
 def test_lcc_bit_exact(counter):
     import torch
 
