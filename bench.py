@@ -151,6 +151,91 @@ def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
             "kernel_cfg": cfg, "autotuned": tuned, "captured": "hipGraph"}
 
 
+def run_serve(tc, pool: int = 48, per_rate: int = 192, rel_rates=(0.25, 0.5, 0.75, 0.9),
+              budget: int = 32 << 30, slots: int = 2, max_batch: int = 64, seed: int = 11) -> dict:
+    """The serving loop under Poisson arrivals: R-MAT requests with n
+    log-uniform in [2^10, 2^20] and edge factor uniform in 8..32 (ids >= n
+    dropped), a 32 GiB admission budget.  First a closed burst of 2*pool
+    requests gives the saturation rate; then open-loop Poisson arrivals at
+    fractions of it give latency percentiles (submit to results on the host,
+    measured by the server).  Every served result is checked against the
+    same graph counted alone."""
+    import threading
+
+    from gputrianglecounting_amd import graph as G
+    from gputrianglecounting_amd import serve as S
+
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for i in range(pool):
+        n = int(np.exp(rng.uniform(np.log(1024), np.log(1 << 20))))
+        ef = int(rng.integers(8, 33))
+        graphs.append(G.rmat(int(np.ceil(np.log2(n))), ef, seed=1000 + i, n_keep=n))
+    arcs = np.array([g.nnz for g in graphs])
+    log(f"serve pool: {pool} graphs, arcs median {int(np.median(arcs))}, max {arcs.max()}")
+    alone = [tc.count_graph(g) for g in graphs]
+    srv = S.Server(hbm_budget=budget, slots=slots, max_batch_graphs=max_batch)
+    bad = 0
+
+    def check(k, r):
+        nonlocal bad
+        if r.total != alone[k][1] or not np.array_equal(r.out, alone[k][0]):
+            bad += 1
+
+    try:
+        # warm the slots' contexts
+        for k in range(min(4, pool)):
+            check(k, srv.wait(srv.submit(graphs[k])))
+        order = [int(x) for x in rng.integers(0, pool, 2 * pool)]
+        t0 = time.perf_counter()
+        ids = [srv.submit(graphs[k]) for k in order]
+        res = [srv.wait(i) for i in ids]
+        t_burst = time.perf_counter() - t0
+        for k, r in zip(order, res):
+            check(k, r)
+        sat = len(order) / t_burst
+        log(f"serve burst: {len(order)} requests in {t_burst * 1e3:.0f} ms = {sat:.1f} graphs/s")
+        rates = []
+        for rel in rel_rates:
+            lam = rel * sat
+            order = [int(x) for x in rng.integers(0, pool, per_rate)]
+            gaps = rng.exponential(1.0 / lam, per_rate)
+            ids = [None] * per_rate
+
+            def producer():
+                t_next = time.perf_counter()
+                for j, k in enumerate(order):
+                    t_next += gaps[j]
+                    dt = t_next - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids[j] = srv.submit(graphs[k])
+
+            th = threading.Thread(target=producer)
+            t1 = time.perf_counter()
+            th.start()
+            th.join()
+            lat = []
+            for j, k in enumerate(order):
+                r = srv.wait(ids[j])
+                check(k, r)
+                lat.append(r.latency_ms)
+            span = time.perf_counter() - t1
+            lat = np.array(lat)
+            rates.append({"offered_graphs_per_s": round(lam, 2), "achieved_graphs_per_s": round(per_rate / span, 2),
+                          "requests": per_rate, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                          "mean_ms": round(float(lat.mean()), 3)})
+            log(f"serve at {lam:.1f}/s: p50 {rates[-1]['p50_ms']} ms, p99 {rates[-1]['p99_ms']} ms")
+        st = srv.stats()
+    finally:
+        srv.close()
+    return {"saturation_graphs_per_s": sat, "burst_requests": 2 * pool, "burst_ms": t_burst * 1e3, "rates": rates,
+            "results_identical_to_alone": bad == 0, "pool_arcs_median": int(np.median(arcs)),
+            "pool_arcs_max": int(arcs.max()), "hbm_budget": budget, "slots": slots, "max_batch_graphs": max_batch,
+            "peak_arena_bytes": st["peak_bytes"], "batches": st["batches"]}
+
+
 def traffic_from_profile(workload: str):
     """HBM bytes per count step (both count kernels) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
@@ -167,7 +252,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er", "batch"])
+    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er", "batch", "serve"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=8,ring=4096")
@@ -191,6 +276,18 @@ def main() -> int:
                           "scaling": "weak", "vs_baseline": None, "dtype": "int32",
                           "data": "synthetic: 4096 seeded ER/R-MAT/RGG graphs",
                           "config": {"workload": "batch_4096_mixed"}, "batched": res}), flush=True)
+        tc.close()
+        return 0
+
+    if args.workload == "serve":
+        tc = TriangleCounter(0)
+        res = run_serve(tc)
+        print(json.dumps({"metric": "serving graphs/s at saturation, p50/p99 latency under Poisson arrivals",
+                          "value": res["saturation_graphs_per_s"], "unit": "graphs/s", "n_gpus": 1,
+                          "steps": res["burst_requests"], "warmup": 4, "ms_per_step": res["burst_ms"] / res["burst_requests"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+                          "data": "synthetic: 48 seeded R-MAT graphs, n log-uniform in [2^10, 2^20], ef 8..32",
+                          "config": {"workload": "serve_rmat_1k_1m_poisson"}, "serve": res}), flush=True)
         tc.close()
         return 0
 

@@ -45,7 +45,7 @@ tc_cfg builtin_default() {
   c.wg_size = 256;
   c.items = 8;
   c.ring = 4096;
-  c.small_max = 2048;
+  c.small_max = 256;
   c.ranges_per_wg = 32;
   c.chunk = 16;
   c.depth = 2;
@@ -550,9 +550,9 @@ std::vector<tc_cfg> tune_candidates() {
     const tcd::Variant v = tcd::variant(i);
     if (v.algo != 0) continue;  // the tuner searches the default (streamed) kernel
     const int lanecap = v.wg_size * v.items;
-    const int sms[3] = {256, 1024, lanecap};
+    const int sms[3] = {64, 256, 1024};
     for (int si = 0; si < 3; ++si) {
-      if (si < 2 && sms[si] >= lanecap) continue;
+      if (sms[si] > lanecap) continue;
       for (int rp : {8, 32, 128}) {
         tc_cfg c{};
         c.wg_size = v.wg_size;

@@ -164,7 +164,7 @@ int tc_graph_destroy(tc_graph* g);
 
 /* ---------------------------------------------------------------- autotuner
  * Candidates: every compiled kernel variant (wg_size, items, ring) times
- * small_max in {256, 1024, wg_size*items} times ranges_per_wg in
+ * small_max in {64, 256, 1024} (at most wg_size*items) times ranges_per_wg in
  * {8, 32, 128}.  Each candidate runs once, its per-vertex output is
  * compared with the first candidate's (identical must hold: the counts are
  * exact integers), then it is timed over reps launches with HIP events.

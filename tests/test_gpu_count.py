@@ -40,7 +40,7 @@ def test_random_graphs(counter, g):
 
 
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: "x".join(map(str, v)))
-@pytest.mark.parametrize("small_max", [16, 300, 0])
+@pytest.mark.parametrize("small_max", [16, 300, 2048, 0])
 @pytest.mark.parametrize("ranges", [1, 32])
 def test_variants(counter, v, small_max, ranges):
     g = G.rmat(13, 16, seed=21)
