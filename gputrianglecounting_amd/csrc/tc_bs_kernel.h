@@ -31,6 +31,9 @@
 //   3. small   whole edges of at most small_max work inside the range,
 //              packed up to M to a tile, Merge Path over the staged lists
 //
+// Edges that edge_search() classifies (B much longer than A) are left to
+// k_count_search by phases 1 and 2.
+//
 // The phases are compiled into separate kernels (PH bit 1 = phase 1, bit 2
 // = phase 2, bit 4 = phase 3) so the hot wave phase gets the register and
 // LDS budget of its own code; both launches walk the same ranges and split
@@ -146,7 +149,8 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         const int32_t la = (int32_t)(a.voff[u + 1] - ab), lb = (int32_t)(a.voff[w + 1] - bb);
         const bool a_is_u = la <= lb;
         const int32_t lA = a_is_u ? la : lb, lB = a_is_u ? lb : la;
-        if (lA > a.wave_acap || lB > BWCAP) continue;  // phase 2
+        if (edge_search(lA, lB, a.search_ratio)) continue;  // k_count_search
+        if (lA > a.wave_acap || lB > BWCAP) continue;       // phase 2
         const int64_t gA = a_is_u ? ab : bb, gB = a_is_u ? bb : ab;
         for (int32_t x0 = 0; x0 < lA; x0 += 8 * 64) {
           int32_t v[8];
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
           const int32_t u = a.eu[ee], w = a.ew[ee];
           const int64_t la = a.voff[u + 1] - a.voff[u], lb = a.voff[w + 1] - a.voff[w];
           const int64_t lA = la < lb ? la : lb, lB = la < lb ? lb : la;
-          mine = lA > a.wave_acap || lB > BWCAP;
+          mine = !edge_search((int32_t)lA, (int32_t)lB, a.search_ratio) && (lA > a.wave_acap || lB > BWCAP);
         }
       }
       const uint64_t bal = __ballot(mine);

@@ -50,6 +50,7 @@ tc_cfg builtin_default() {
   c.chunk = 16;
   c.depth = 2;
   c.algo = 0;
+  c.search_ratio = 64;
   return c;
 }
 
@@ -65,6 +66,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->ranges_per_wg) c.ranges_per_wg = in->ranges_per_wg;
     if (in->chunk) c.chunk = in->chunk;
     if (in->depth) c.depth = in->depth;
+    if (in->search_ratio) c.search_ratio = in->search_ratio;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -108,7 +110,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
   L.ework = take(4 * (size_t)L.ecap);
   L.wpre = take(8 * (size_t)(L.ecap + 1));
   L.dscal = take(8 * 8);
-  L.qctr = take(4 * 16);
+  L.qctr = take(4 * 24);
   const int64_t ncap = std::max<int64_t>(nv + 1, L.ecap);
   L.scan = take(8 * tcd::scan_tmp_elems(ncap));
   L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
@@ -186,9 +188,12 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.small_max = cfg.small_max;
   a.min_units = 4 * cfg.wg_size * cfg.items;
   a.qctr = p.qctr;
+  a.search_ratio = cfg.algo == 0 && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
-  hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, count_wg(c, cfg, vi, 1), s);
+  int nwg3 = tcd::resident_search_wgs(c->cus);
+  if (nwg3 >= 64) nwg3 &= ~7;
+  hipError_t e = tcd::launch_count(a, cfg.algo, vi, nwg, count_wg(c, cfg, vi, 1), nwg3 > 0 ? nwg3 : 1, s);
   if (e != hipSuccess) return hip_fail(c, e, "count kernel launch");
   return TC_OK;
 }

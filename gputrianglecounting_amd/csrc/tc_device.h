@@ -25,8 +25,9 @@ struct CountArgs {
   int32_t small_max;
   int32_t max_ranges;           // level-1 ranges: min(W / min_units, max_ranges)
   int32_t min_units;            // smallest level-1 range
-  unsigned int* qctr;           // 8 per-XCD range queue counters, zeroed per launch
+  unsigned int* qctr;           // 8 per-XCD range queue counters, zeroed per count
   int32_t wave_acap;            // streamed kernel: longest shorter-list a single wave stages
+  int32_t search_ratio;         // edges with |B| >= search_ratio * |A| go to k_count_search (0: none)
 };
 
 // Kernel variants compiled into the library.
@@ -47,7 +48,7 @@ struct Prep {
   int32_t* ework;   // [ecap]
   int64_t* wpre;    // [ecap + 1]
   int64_t* dscal;   // [8]: E, W, overflow flag, -, two vertex-chunk counters
-  unsigned int* qctr;  // [16]: two launches' per-XCD range queues
+  unsigned int* qctr;  // [24]: three launches' per-XCD range queues
   int64_t* scan_tmp;
   int64_t ecap;
   // oriented copy for the total-only mode
@@ -71,7 +72,10 @@ hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus,
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
 int resident_wgs2(int variant_idx, int cus, int which);
-hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, hipStream_t s);
+// nwg, nwg2, nwg3: grids of the streamed kernel's wave launch, its
+// workgroup launch and the lookup kernel (persistent: resident workgroups).
+hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
+int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,
                            int cus, hipStream_t s);

@@ -453,7 +453,18 @@ __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ bas
   return cnt;
 }
 
+// Edges whose longer list B is at least `ratio` times the shorter list A
+// (and at least kSearchMinB long) are counted by k_count_search, looking each
+// element of A up in B, instead of streaming B.  Small whole edges (work at
+// most small_max, wholly inside one range) stay with the tiles whatever their
+// shape; every count kernel applies the same two tests in this order.
+constexpr int32_t kSearchMinB = 512;
+__device__ __forceinline__ bool edge_search(int32_t lA, int32_t lB, int32_t ratio) {
+  return ratio > 0 && lB >= kSearchMinB && (int64_t)lB >= (int64_t)ratio * lA;
+}
+
 #include "tc_bs_kernel.h"
+#include "tc_search_kernel.h"
 
 // One lane per edge, two-pointer merge from global memory.  Simple and
 // unbalanced; kept as an independent cross-check of k_count_mp.
@@ -699,22 +710,38 @@ int resident_wgs2(int vi, int cus, int which) {
 
 int resident_wgs(int vi, int cus) { return resident_wgs2(vi, cus, 0); }
 
-hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, int nwg2, hipStream_t s) {
+constexpr int kSearchBS = 256;
+int resident_search_wgs(int cus) {
+  static std::mutex mu;
+  static int occ = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!occ) occ = occupancy(k_count_search<kSearchBS>, kSearchBS);
+  return occ * cus;
+}
+
+hipError_t launch_count(const CountArgs& a, int algo, int vi, int nwg, int nwg2, int nwg3, hipStream_t s) {
   if (algo == 1) {
-    hipLaunchKernelGGL(k_count_simple, dim3(nwg), dim3(256), 0, s, a);
+    CountArgs b = a;
+    b.search_ratio = 0;
+    hipLaunchKernelGGL(k_count_simple, dim3(nwg), dim3(256), 0, s, b);
     return hipGetLastError();
   }
   if (vi < 0 || vi >= kNumVariants) return hipErrorInvalidValue;
   const VariantDef& v = kVariants[vi];
-  hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.qctr), 16, 0, s);
+  hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.qctr), 24, 0, s);
   if (e != hipSuccess) return e;
   CountArgs b = a;
   b.wave_acap = v.awt;
+  if (!v.fn2) b.search_ratio = 0;  // the ring kernel counts every edge itself
   hipLaunchKernelGGL(v.fn, dim3(nwg), dim3(v.bs), 0, s, b);
   if (v.fn2) {
     // same ranges (a.max_ranges), its own range queues
     b.qctr = a.qctr + 8;
     hipLaunchKernelGGL(v.fn2, dim3(nwg2 > 0 ? nwg2 : 1), dim3(v.bs), 0, s, b);
+    if (b.search_ratio > 0) {
+      b.qctr = a.qctr + 16;
+      hipLaunchKernelGGL(k_count_search<kSearchBS>, dim3(nwg3 > 0 ? nwg3 : 1), dim3(kSearchBS), 0, s, b);
+    }
   }
   return hipGetLastError();
 }

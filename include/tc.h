@@ -50,7 +50,12 @@ typedef struct tc_ctx tc_ctx;
  *                  per edge (simple, for cross-checks), 2 = Merge Path
  *                  through LDS rings for every long edge
  *   chunk          algo 0: elements per streamed chunk (16 or 32)
- *   depth          algo 0: streamed chunks in flight per lane (2 or 4) */
+ *   depth          algo 0: streamed chunks in flight per lane (2 or 4)
+ *   search_ratio   algo 0: an edge whose longer list B has at least 512 and
+ *                  at least search_ratio times as many elements as the
+ *                  shorter list A is counted by binary-searching every
+ *                  element of A in B instead of streaming B (default 64;
+ *                  negative: never) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
@@ -60,6 +65,7 @@ typedef struct tc_cfg {
   int32_t algo;
   int32_t chunk;
   int32_t depth;
+  int32_t search_ratio;
 } tc_cfg;
 
 enum {

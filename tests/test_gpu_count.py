@@ -49,6 +49,16 @@ def test_variants(counter, v, small_max, ranges):
     _check(counter, g, cfg)
 
 
+@pytest.mark.parametrize("ratio", [1, 4, 64, -1])
+@pytest.mark.parametrize("ranges", [1, 4096])
+def test_search_ratio(counter, ratio, ranges):
+    """Edges with the longer list at least ratio times the shorter (and at
+    least 512 long) are counted by lookups; ratio 1 sends every such edge
+    there, 4096 ranges per workgroup split long edges across ranges."""
+    for g in [G.rmat(13, 32, seed=3), G.rmat(12, 64, seed=4), GR.wheel(3000)]:
+        _check(counter, g, cfg=dict(search_ratio=ratio, ranges_per_wg=ranges))
+
+
 def test_simple_kernel_agrees(counter):
     g = G.rmat(12, 16, seed=4)
     _check(counter, g, dict(algo=1))
