@@ -50,14 +50,52 @@ def make_graph(workload: str):
     raise SystemExit(f"unknown workload {workload}")
 
 
-def algorithmic_bytes(g, W: int) -> dict:
-    """Bytes the count moves by definition: every intersected edge streams both
-    sorted lists once (4 B per element, W elements), plus reading the CSR once
-    and writing int32 counts and the int64 total.  ``b_alg_all`` is the same
-    with every undirected edge intersected (4 * sum d^2)."""
-    d = g.degrees().astype(np.float64)
+SEARCH_MIN_B = 512  # kSearchMinB in tc_kernels.hip
+
+
+def algorithmic_bytes(g, small_max: int, search_ratio: int) -> dict:
+    """Bytes the count moves by definition of the algorithm it runs.
+
+    Every intersected edge (u < w, both degrees >= 2) with shorter list A
+    and longer list B is either streamed, reading both lists once
+    (4 (|A| + |B|) bytes), or, when |B| >= 512 and |B| >= search_ratio |A|
+    (and the edge is not a small one, |A| + |B| > small_max), counted by
+    lookups: every element of A read once and binary-searched in B, one
+    4-byte probe per level (4 |A| (1 + ceil(log2(|B| + 1))) bytes).  Plus
+    reading the CSR once and writing int32 counts and the int64 total.
+    ``b_alg_all`` is the all-streamed figure (4 * sum d^2 + the same fixed
+    part), the formula of a pure merge."""
+    off = g.off.astype(np.int64)
+    ind = g.ind
+    d = np.diff(off)
     fixed = 4 * (g.n + 1) + 4 * g.nnz + 4 * g.n + 8
-    return dict(b_kernel=4 * W + fixed, b_alg_all=4 * float((d * d).sum()) + fixed)
+    stream_b = search_b = 0.0
+    stream_units = 0
+    n_search = n_edges = 0
+    r0 = 0
+    while r0 < g.n:  # row blocks of about 2^25 arcs
+        r1 = int(np.searchsorted(off, off[r0] + (1 << 25), side="right"))
+        r1 = min(max(r1, r0 + 1), g.n)
+        src = np.repeat(np.arange(r0, r1, dtype=np.int64), d[r0:r1])
+        dst = ind[off[r0]:off[r1]].astype(np.int64)
+        m = src < dst
+        du, dw = d[src[m]], d[dst[m]]
+        keep = (du >= 2) & (dw >= 2)
+        a = np.minimum(du, dw)[keep]
+        b = np.maximum(du, dw)[keep]
+        L = a + b
+        srch = (L > small_max) & (b >= SEARCH_MIN_B) & (search_ratio > 0) & (b >= search_ratio * a)
+        stream_units += int(L[~srch].sum())
+        stream_b += 4.0 * float(L[~srch].sum())
+        la = a[srch].astype(np.float64)
+        search_b += float((4.0 * la * (1.0 + np.ceil(np.log2(b[srch].astype(np.float64) + 1.0)))).sum())
+        n_search += int(srch.sum())
+        n_edges += int(keep.sum())
+        r0 = r1
+    dd = d.astype(np.float64)
+    return dict(b_kernel=stream_b + search_b + fixed, b_stream=stream_b, b_search=search_b,
+                b_alg_all=4 * float((dd * dd).sum()) + fixed, search_edges=n_search, edges=n_edges,
+                stream_units=stream_units)
 
 
 def cpu_baseline(g, target_s: float) -> dict:
@@ -342,7 +380,8 @@ def main() -> int:
 
     edges = g.nnz // 2
     value = edges / (ms_per_step / 1e3)
-    ab = algorithmic_bytes(g, W)
+    kcfg = tc.default_cfg() if cfg is None else dict(tc.default_cfg(), **cfg)
+    ab = algorithmic_bytes(g, kcfg["small_max"], kcfg["search_ratio"] if kcfg["algo"] in (0, 3) else -1)
     t_k = float(np.mean(kern)) / 1e3
     achieved = ab["b_kernel"] / t_k / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -376,10 +415,14 @@ def main() -> int:
         "count_kernel_ms": round(float(np.mean(kern)), 3),
         "prep_ms": round(float(np.mean(prep)), 3),
         "b_kernel_bytes": ab["b_kernel"],
+        "b_kernel_streamed_bytes": ab["b_stream"],
+        "b_kernel_lookup_bytes": ab["b_search"],
+        "lookup_edges": ab["search_edges"],
         "b_alg_all_edges_bytes": ab["b_alg_all"],
         "hbm_gbs_all_edges_formula": round(ab["b_alg_all"] / t_k / 1e9, 1),
         "batched": batch,
         "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
+                   "host_edge_count_agrees": ab["edges"] == E,
                    "totals_agree": tot == tot_global, "triangles": tot // 6},
     }
     print(json.dumps(line), flush=True)

@@ -210,8 +210,9 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "cannot write %s\n", fname.c_str());
       return 1;
     }
-    static const int kVar[][6] = {{0, 256, 8, 4096, 16, 2}, {0, 256, 8, 4096, 16, 4}, {0, 512, 4, 4096, 16, 2},
-                                  {0, 128, 16, 4096, 16, 2}, {2, 256, 8, 4096, 0, 0}};
+    static const int kVar[][6] = {{0, 256, 8, 4096, 16, 2}, {0, 256, 8, 4096, 16, 4}, {0, 128, 16, 4096, 16, 2},
+                                  {0, 512, 4, 4096, 16, 2},  {3, 256, 8, 4096, 16, 2}, {3, 256, 8, 4096, 16, 4},
+                                  {3, 512, 4, 4096, 16, 2},  {3, 128, 16, 4096, 16, 2}, {2, 256, 8, 4096, 0, 0}};
     for (const auto& v : kVar) {
       tc_cfg cfg{};
       cfg.algo = v[0];

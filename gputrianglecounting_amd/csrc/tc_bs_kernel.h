@@ -20,6 +20,11 @@
 // edge splits B proportionally: the part covering units [t0, t1) of an edge
 // owns B[t0*|B|/L, t1*|B|/L), so the parts of all ranges tile B exactly.
 //
+// With PH bit 8 the wave phase takes every streamed edge: an A longer than
+// the wave's LDS slice is walked in windows, each window meeting only the B
+// elements of its value range (found by a wave-wide search), and no edge is
+// left to phase 2.
+//
 // Per range, three phases:
 //   1. waves   edges whose shorter list fits a per-wave LDS slice (AW) and
 //              whose longer list is at most BWCAP: one wave per edge, edges
@@ -150,31 +155,59 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         const bool a_is_u = la <= lb;
         const int32_t lA = a_is_u ? la : lb, lB = a_is_u ? lb : la;
         if (edge_search(lA, lB, a.search_ratio)) continue;  // k_count_search
-        if (lA > a.wave_acap || lB > BWCAP) continue;       // phase 2
-        const int64_t gA = a_is_u ? ab : bb, gB = a_is_u ? bb : ab;
-        for (int32_t x0 = 0; x0 < lA; x0 += 8 * 64) {
-          int32_t v[8];
-#pragma unroll
-          for (int r = 0; r < 8; ++r) {
-            const int32_t x = x0 + lane + r * 64;
-            v[r] = x < lA ? a.ind[gA + x] : 0;
-          }
-#pragma unroll
-          for (int r = 0; r < 8; ++r) {
-            const int32_t x = x0 + lane + r * 64;
-            if (x < lA) abuf[x] = v[r];
-          }
+        if constexpr ((PH & 8) == 0) {
+          if (lA > a.wave_acap || lB > BWCAP) continue;  // phase 2
         }
-        if (lane == 0) abuf[lA] = INT_MAX;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int64_t gA = a_is_u ? ab : bb, gB = a_is_u ? bb : ab;
+        // stage A[x0, x0 + n) (n <= AW) into abuf[0, n) with abuf[n] = INT_MAX
+        auto stage = [&](int64_t x0, int32_t n) {
+          for (int32_t y0 = 0; y0 < n; y0 += 8 * 64) {
+            int32_t v[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+              const int32_t y = y0 + lane + r * 64;
+              v[r] = y < n ? a.ind[gA + x0 + y] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+              const int32_t y = y0 + lane + r * 64;
+              if (y < n) abuf[y] = v[r];
+            }
+          }
+          if (lane == 0) abuf[n] = INT_MAX;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        // B elements at ind4 positions [j0, j1) against the staged A
+        auto stream = [&](int64_t j0, int64_t j1, int32_t n) -> int32_t {
+          const int64_t cb = j0 / CH, nc = (j1 + CH - 1) / CH - cb;
+          const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
+          return stream_chunks<CH, SD>(ind4, c_lo, c_hi, j0, j1, abuf, n);
+        };
         const int64_t g0 = gB + sh + (t0 * lB) / L, g1 = gB + sh + (t1 * lB) / L;
         int32_t c = 0;
-        if (g0 < g1) {
-          const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
-          const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
-          c = stream_chunks<CH, SD>(ind4, c_lo, c_hi, g0, g1, abuf, lA);
+        if (lA <= AW) {
+          stage(0, lA);
+          if (g0 < g1) c = stream(g0, g1, lA);
+        } else if (g0 < g1) {
+          // A longer than the wave's slice (only with PH & 8): A in windows
+          // of AW.  Only A[ia, ia_end) can match B's part; window A[ia, ia+n)
+          // meets exactly the B elements from jb up to the first one above
+          // its last element, and the next window starts after both.
+          const int32_t* Ag = a.ind + gA;
+          int64_t ia = wave_lower_bound(Ag, 0, lA, ind4[g0], lane);
+          const int64_t ia_end = wave_lower_bound(Ag, ia, lA, ind4[g1 - 1] + 1, lane);
+          int64_t jb = g0;
+          while (ia < ia_end && jb < g1) {
+            const int32_t n = (int32_t)(ia_end - ia < AW ? ia_end - ia : AW);
+            stage(ia, n);
+            const int64_t je = ia + n >= ia_end ? g1 : wave_lower_bound(ind4, jb, g1, Ag[ia + n - 1] + 1, lane);
+            if (jb < je) c += stream(jb, je, n);
+            jb = je;
+            ia += n;
+            __builtin_amdgcn_wave_barrier();  // abuf is rewritten by the next window
+          }
         }
         c = wave_sum32(c);
         if (lane == 0 && c) {

@@ -73,10 +73,11 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
   for (int i = 0; i < tcd::num_variants(); ++i) {
     tcd::Variant v = tcd::variant(i);
     const int want = c.algo == 1 ? 0 : c.algo;  // the simple kernel reuses the streamed sizes
-    const bool stream_ok = want != 0 || (v.chunk == c.chunk && v.depth == c.depth);
+    const bool streamed = want == 0 || want == 3;
+    const bool stream_ok = !streamed || (v.chunk == c.chunk && v.depth == c.depth);
     if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want && stream_ok) vi = i;
   }
-  if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 2) return -1;
+  if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 3) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   *out = c;
@@ -188,7 +189,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.small_max = cfg.small_max;
   a.min_units = 4 * cfg.wg_size * cfg.items;
   a.qctr = p.qctr;
-  a.search_ratio = cfg.algo == 0 && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
+  a.search_ratio = (cfg.algo == 0 || cfg.algo == 3) && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
   int nwg3 = tcd::resident_search_wgs(c->cus);
@@ -553,7 +554,7 @@ std::vector<tc_cfg> tune_candidates() {
   std::vector<tc_cfg> out;
   for (int i = 0; i < tcd::num_variants(); ++i) {
     const tcd::Variant v = tcd::variant(i);
-    if (v.algo != 0) continue;  // the tuner searches the default (streamed) kernel
+    if (v.algo != 0 && v.algo != 3) continue;  // the tuner searches the streamed kernels
     const int lanecap = v.wg_size * v.items;
     const int sms[3] = {64, 256, 1024};
     for (int si = 0; si < 3; ++si) {

@@ -453,6 +453,30 @@ __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ bas
   return cnt;
 }
 
+// First position p in [lo, hi) with X[p] >= key (hi if there is none),
+// found by the whole wave together: each round probes 64 evenly spaced
+// positions with one load per lane, and the ballot of X < key (a prefix of
+// the lanes, X being sorted) leaves the gap after the last smaller probe.
+// The answer stays in [lo, hi] throughout; every load is inside [lo, hi).
+// Wave-uniform arguments and result.
+__device__ __forceinline__ int64_t wave_lower_bound(const int32_t* __restrict__ X, int64_t lo, int64_t hi, int32_t key,
+                                                    int lane) {
+  while (hi - lo > 64) {
+    const int64_t step = (hi - lo + 63) >> 6;
+    const int64_t q = lo + (int64_t)lane * step;
+    const bool less = q < hi && X[q] < key;
+    const int c = __popcll(__ballot(less));
+    const int64_t qc = lo + (int64_t)c * step;  // first probe not smaller (or past hi)
+    const int64_t nlo = c ? lo + (int64_t)(c - 1) * step + 1 : lo;
+    const int64_t nhi = c ? (qc < hi ? qc : hi) : lo;
+    lo = uniform64(nlo);
+    hi = uniform64(nhi);
+  }
+  const int64_t q = lo + lane;
+  const bool less = q < hi && X[q] < key;
+  return lo + __popcll(__ballot(less));
+}
+
 // Edges whose longer list B is at least `ratio` times the shorter list A
 // (and at least kSearchMinB long) are counted by k_count_search, looking each
 // element of A up in B, instead of streaming B.  Small whole edges (work at
@@ -584,19 +608,29 @@ unsigned grid_for(int64_t n, int bs, int64_t cap = 65536) {
   return (unsigned)g;
 }
 
-// Compiled count kernels: algo 0 = streamed (k_count_bs, launched as a wave
-// kernel fn then a workgroup/small-tile kernel fn2), 2 = Merge Path rings
-// only (k_count_mp).  {algo, BS, K, P, M, chunk, depth, wave A cap}.
+// Compiled count kernels: algo 0 = streamed (k_count_bs) with every long
+// edge on one wave (A in windows), launched as that wave kernel fn then a
+// small-tile kernel fn2; 3 = streamed with the longest edges left to a
+// workgroup/ring phase in fn2; 2 = Merge Path rings only (k_count_mp).
+// Algos 0 and 3 also launch k_count_search.
+// {algo, BS, K, P, M, chunk, depth, wave A cap}.
 struct VariantDef {
   int algo, bs, k, p, m, ch, sd, awt;
   void (*fn)(CountArgs);
   void (*fn2)(CountArgs);
 };
+#define TC_BW(BS, CH, P, M, K, SD, AWT)                                                        \
+  {0, BS, K, P, M, CH, SD, AWT, k_count_bs<BS, CH, P, M, K, SD, 9, AWT>,                    \
+   k_count_bs<BS, CH, P, M, K, SD, 4, AWT>}
 #define TC_BS(BS, CH, P, M, K, SD, AWT)                                                        \
-  {0, BS, K, P, M, CH, SD, AWT, k_count_bs<BS, CH, P, M, K, SD, 1, AWT>,                    \
+  {3, BS, K, P, M, CH, SD, AWT, k_count_bs<BS, CH, P, M, K, SD, 1, AWT>,                    \
    k_count_bs<BS, CH, P, M, K, SD, 6, AWT>}
 #define TC_MP(BS, K, P, M) {2, BS, K, P, M, 0, 0, 0, k_count_mp<BS, K, P, M>, nullptr}
 const VariantDef kVariants[] = {
+    TC_BW(256, 16, 4096, 128, 8, 2, 2039),
+    TC_BW(256, 16, 4096, 128, 8, 4, 2039),
+    TC_BW(128, 16, 4096, 64, 16, 2, 4087),
+    TC_BW(512, 16, 4096, 128, 4, 2, 1019),
     TC_BS(256, 16, 4096, 128, 8, 2, 2039),
     TC_BS(256, 16, 4096, 128, 8, 4, 2039),
     TC_BS(512, 16, 4096, 128, 4, 2, 1019),
@@ -609,6 +643,7 @@ const VariantDef kVariants[] = {
     TC_MP(512, 4, 4096, 128),
 };
 #undef TC_BS
+#undef TC_BW
 #undef TC_MP
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

@@ -46,12 +46,15 @@ typedef struct tc_ctx tc_ctx;
  *   ranges_per_wg  level-1 split: the edge-work range [0, W) is cut into
  *                  ranges_per_wg * (resident workgroups) equal ranges that a
  *                  persistent grid pulls from per-XCD queues
- *   algo           0 = streamed two-level kernel (default), 1 = one lane
- *                  per edge (simple, for cross-checks), 2 = Merge Path
- *                  through LDS rings for every long edge
- *   chunk          algo 0: elements per streamed chunk (16 or 32)
- *   depth          algo 0: streamed chunks in flight per lane (2 or 4)
- *   search_ratio   algo 0: an edge whose longer list B has at least 512 and
+ *   algo           0 = streamed kernel with every long edge on one wave
+ *                  (the shorter list walked in LDS windows) plus packed
+ *                  small-edge tiles (default); 3 = streamed kernel whose
+ *                  longest edges go to workgroup streams and LDS rings;
+ *                  1 = one lane per edge (simple, for cross-checks);
+ *                  2 = Merge Path through LDS rings for every long edge
+ *   chunk          algo 0/3: elements per streamed chunk (16)
+ *   depth          algo 0/3: streamed chunks in flight per lane (2 or 4)
+ *   search_ratio   algo 0/3: an edge whose longer list B has at least 512 and
  *                  at least search_ratio times as many elements as the
  *                  shorter list A is counted by binary-searching every
  *                  element of A in B instead of streaming B (default 64;

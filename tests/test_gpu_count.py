@@ -14,16 +14,28 @@ from tests import graphs as GR
 
 pytestmark = pytest.mark.gpu
 
-# (algo, wg_size, items, ring): algo 0 = streamed kernel, 2 = Merge Path rings only
-VARIANTS = [(0, 256, 8, 4096, 16, 2), (0, 256, 8, 4096, 16, 4), (0, 512, 4, 4096, 16, 2), (0, 128, 16, 4096, 16, 2),
+# (algo, wg_size, items, ring, chunk, depth): algo 0 = streamed kernel with A windows, 3 = streamed with
+# workgroup/ring phase, 2 = Merge Path rings only
+VARIANTS = [(0, 256, 8, 4096, 16, 2), (0, 256, 8, 4096, 16, 4), (0, 128, 16, 4096, 16, 2), (0, 512, 4, 4096, 16, 2),
+            (3, 256, 8, 4096, 16, 2), (3, 256, 8, 4096, 16, 4), (3, 512, 4, 4096, 16, 2), (3, 128, 16, 4096, 16, 2),
             (2, 256, 8, 4096, 0, 0), (2, 256, 16, 8192, 0, 0), (2, 512, 8, 8192, 0, 0), (2, 128, 16, 4096, 0, 0),
             (2, 256, 4, 2048, 0, 0), (2, 512, 4, 4096, 0, 0)]
+
+
+_REF = {}
+
+
+def _ref(g):
+    key = (g.name, g.n, g.nnz)
+    if key not in _REF:
+        _REF[key] = oracle.per_vertex(g.off, g.ind)
+    return _REF[key]
 
 
 def _check(counter, g, cfg=None):
     print(f"case {g.name} n={g.n} nnz={g.nnz} cfg={cfg}", flush=True)
     out, tot = counter.count_graph(g, cfg=cfg)
-    ref, rtot = oracle.per_vertex(g.off, g.ind)
+    ref, rtot = _ref(g)
     assert out.dtype == np.int32 and out.shape == (g.n,)
     np.testing.assert_array_equal(out, ref)
     assert tot == rtot == int(out.astype(np.int64).sum())
@@ -49,14 +61,27 @@ def test_variants(counter, v, small_max, ranges):
     _check(counter, g, cfg)
 
 
+_SG = []
+
+
+def _search_graphs():
+    if not _SG:
+        _SG.extend([G.rmat(13, 32, seed=3), G.rmat(13, 64, seed=4), GR.wheel(3000)])
+    return _SG
+
+
+@pytest.mark.parametrize("algo", [0, 3])
 @pytest.mark.parametrize("ratio", [1, 4, 64, -1])
 @pytest.mark.parametrize("ranges", [1, 4096])
-def test_search_ratio(counter, ratio, ranges):
+def test_search_ratio(counter, algo, ratio, ranges):
     """Edges with the longer list at least ratio times the shorter (and at
     least 512 long) are counted by lookups; ratio 1 sends every such edge
-    there, 4096 ranges per workgroup split long edges across ranges."""
-    for g in [G.rmat(13, 32, seed=3), G.rmat(12, 64, seed=4), GR.wheel(3000)]:
-        _check(counter, g, cfg=dict(search_ratio=ratio, ranges_per_wg=ranges))
+    there, -1 none; 4096 ranges per workgroup split long edges across
+    ranges.  R-MAT 13/64 has 14 hubs of degree above 2039, so hub-hub edges
+    have a shorter list longer than one wave's LDS slice (algo 3 walks it
+    in windows)."""
+    for g in _search_graphs():
+        _check(counter, g, cfg=dict(algo=algo, search_ratio=ratio, ranges_per_wg=ranges))
 
 
 def test_simple_kernel_agrees(counter):
