@@ -462,6 +462,46 @@ tch_graph* tch_read_info(const char* path_no_ext) {
   return g;
 }
 
+int tch_read_partition(const char* path_no_ext, int32_t K, int32_t* n, int32_t** vals) {
+  if (!path_no_ext || K < 1 || !n || !vals) return fail(TCH_EINVAL, "invalid argument");
+  *n = 0;
+  *vals = nullptr;
+  const std::string path = std::string(path_no_ext) + ".part." + std::to_string(K);
+  std::string buf;
+  if (!slurp(path.c_str(), buf)) return fail(TCH_EIO, "cannot read " + path);
+  Scanner sc{buf.data(), buf.data() + buf.size()};
+  int64_t cnt, v;
+  if (!sc.next(cnt) || cnt < 0 || cnt > INT32_MAX) return fail(TCH_EIO, path + ": bad header");
+  int32_t* a = static_cast<int32_t*>(std::malloc(4 * (size_t)(cnt > 0 ? cnt : 1)));
+  if (!a) return fail(TCH_ENOMEM, "out of memory");
+  for (int64_t i = 0; i < cnt; ++i) {
+    if (!sc.next(v) || v < INT32_MIN || v > INT32_MAX) {
+      std::free(a);
+      return fail(TCH_EIO, path + ": truncated or out of range");
+    }
+    a[i] = (int32_t)v;
+  }
+  *n = (int32_t)cnt;
+  *vals = a;
+  return TCH_OK;
+}
+
+int tch_partition_members(int32_t n, const int32_t* part, int32_t p, int32_t* count, int32_t** ids) {
+  if (n < 0 || (n > 0 && !part) || !count || !ids) return fail(TCH_EINVAL, "invalid argument");
+  int32_t c = 0;
+  for (int32_t v = 0; v < n; ++v) c += part[v] == p;
+  int32_t* a = static_cast<int32_t*>(std::malloc(4 * (size_t)(c > 0 ? c : 1)));
+  if (!a) return fail(TCH_ENOMEM, "out of memory");
+  int32_t k = 0;
+  for (int32_t v = 0; v < n; ++v)
+    if (part[v] == p) a[k++] = v;
+  *count = c;
+  *ids = a;
+  return TCH_OK;
+}
+
+void tch_free(void* p) { std::free(p); }
+
 int tch_write_info(const tch_graph* g, const char* path) {
   if (!g || !path) return fail(TCH_EINVAL, "invalid argument");
   FILE* f = std::fopen(path, "wb");

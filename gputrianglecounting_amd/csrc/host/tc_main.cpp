@@ -87,7 +87,7 @@ tch_graph* load(const std::string& spec) {
 
 void usage() {
   std::fprintf(stderr,
-               "usage: tc_main <graph> <run> <blocks> [<blockSize> <threadsPerIntersection>]\n"
+               "usage: tc_main <graph> <run> <blocks> [<blockSize> <threadsPerIntersection> [<partitions>]]\n"
                "  <graph>: path without .info | gen:er:n:m:seed | gen:rmat:scale:ef:seed |\n"
                "           gen:rgg:n:deg:seed | file.bin | file.graph\n");
 }
@@ -188,7 +188,42 @@ int main(int argc, char** argv) {
   const double alloc_ms = ms_since(t0);
 
   int rc = 0;
-  if (single) {
+  if (single && argc > 6) {
+    // Partition run (param.cuh PAR_PARTITION): <graph>.part.K holds a METIS
+    // partition vector; each partition's vertices are counted as one vertex
+    // list (the reference's vertex-list kernelCall) and the totals summed.
+    const int32_t K = std::atoi(argv[6]);
+    int32_t npart = 0, *part = nullptr;
+    if (K < 1 || tch_read_partition(spec.c_str(), K, &npart, &part) != TCH_OK || npart != nv) {
+      std::fprintf(stderr, "partition file: %s\n", K < 1 ? "bad partition count" : npart != nv ? "length differs from nv" : tch_last_error());
+      return 1;
+    }
+    int32_t* d_list = nullptr;
+    HIPCHECK(hipMalloc(&d_list, 4 * std::max<size_t>((size_t)nv, 1)));
+    int64_t* d_pt = nullptr;
+    HIPCHECK(hipMalloc(&d_pt, 8));
+    auto t1 = clk::now();
+    int64_t total = 0;
+    for (int32_t p = 0; p < K && rc == TC_OK; ++p) {
+      int32_t cnt = 0, *ids = nullptr;
+      if (tch_partition_members(nv, part, p, &cnt, &ids) != TCH_OK) return 1;
+      if (cnt) HIPCHECK(hipMemcpy(d_list, ids, 4 * (size_t)cnt, hipMemcpyHostToDevice));
+      rc = tc_count_vertices_i32(ctx, nv, nnz, d_off, d_ind, cnt, d_list, d_out, d_pt, nullptr, nullptr);
+      int64_t pt = 0;
+      HIPCHECK(hipMemcpy(&pt, d_pt, 8, hipMemcpyDeviceToHost));
+      total += pt;
+      tch_free(ids);
+    }
+    const double count_ms = ms_since(t1);
+    tch_free(part);
+    (void)hipFree(d_list);
+    (void)hipFree(d_pt);
+    if (rc != TC_OK) {
+      std::fprintf(stderr, "count failed: %s\n", tc_last_error(ctx));
+      return 1;
+    }
+    std::printf("%d\t%lld\t%g\t%g\t%g\n", nv, (long long)total, alloc_ms, count_ms, ms_since(t0));
+  } else if (single) {
     auto t1 = clk::now();
     rc = tc_count_i32(ctx, nv, nnz, d_off, d_ind, d_out, d_total, nullptr, nullptr);
     HIPCHECK(hipDeviceSynchronize());

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
         if (lane == 0 && c) {
           if (a.out) {
             atomicAdd(&a.out[u], c);
-            atomicAdd(&a.out[w], c);
+            if (!a.u_only) atomicAdd(&a.out[w], c);
           }
           cnt_w += c;
         }
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
           if (tid == 0 && c) {
             if (a.out) {
               atomicAdd(&a.out[u], (int32_t)c);
-              atomicAdd(&a.out[w], (int32_t)c);
+              if (!a.u_only) atomicAdd(&a.out[w], (int32_t)c);
             }
             my_sum += c;
           }
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
       if (tid < c) {
         const int32_t cc = m_cnt[tid];
         if (cc) {
-          if (a.out) atomicAdd(&a.out[m_w[tid]], cc);
+          if (a.out && !a.u_only) atomicAdd(&a.out[m_w[tid]], cc);
           my_sum += cc;
         }
         const int32_t u = m_u[tid];

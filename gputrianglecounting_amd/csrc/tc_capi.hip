@@ -86,11 +86,12 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 
 // Scratch layout shared by the single-graph, batch and total-only paths.
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, vmark, vcnt, total;
   int64_t ecap, ocap;
+  bool list;
 };
 
-Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
+Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = false) {
   Layout L{};
   size_t p = 0;
   auto take = [&](size_t bytes) {
@@ -99,8 +100,10 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
     return at;
   };
   // Undirected edges: at most nnz/2 for a symmetric CSR.  In the oriented
-  // mode every kept arc is an edge, again at most nnz/2.
-  L.ecap = nnz / 2 + 1;
+  // mode every kept arc is an edge, again at most nnz/2; in the vertex-list
+  // mode every arc of a listed vertex is one, at most nnz.
+  L.list = list;
+  L.ecap = list ? nnz + 1 : nnz / 2 + 1;
   L.ocap = orient ? nnz / 2 + 1 : 0;
   L.voff = take(8 * (size_t)(nv + 1));
   L.vbase = take(batch ? 4 * (size_t)(nv + 1) : 0);
@@ -116,6 +119,8 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient) {
   L.scan = take(8 * tcd::scan_tmp_elems(ncap));
   L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
   L.oind = take(orient ? 4 * (size_t)L.ocap : 0);
+  L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
+  L.vcnt = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
   return L;
 }
@@ -157,6 +162,8 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.ecap = L.ecap;
   p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
   p.oind = L.ocap ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
+  p.vmark = L.list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
+  p.vcnt = L.list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
 }
 
@@ -175,7 +182,7 @@ int count_wg(const tc_ctx* c, const tc_cfg& cfg, int vi, int which = 0) {
 
 // Edge list (already built in p) -> counts.
 int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const int32_t* ind, int32_t* out,
-              int64_t* total, int32_t mult, hipStream_t s) {
+              int64_t* total, int32_t mult, hipStream_t s, bool u_only = false) {
   tcd::CountArgs a{};
   a.wpre = p.wpre;
   a.eu = p.eu;
@@ -190,6 +197,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.min_units = 4 * cfg.wg_size * cfg.items;
   a.qctr = p.qctr;
   a.search_ratio = (cfg.algo == 0 || cfg.algo == 3) && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
+  a.u_only = u_only ? 1 : 0;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
   int nwg3 = tcd::resident_search_wgs(c->cus);
@@ -373,6 +381,43 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if (rc) return rc;
   c->last_dscal = p.dscal;
   record(c, 2, s);
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
+int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                          int32_t nlist, const int32_t* d_list, int32_t* d_out, int64_t* d_total, const tc_cfg* cfg_in,
+                          void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || nlist < 0 || !d_off || (nnz > 0 && !d_ind) || (nlist > 0 && (!d_list || !d_out)))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv, nnz, false, false, true);
+  int rc = ensure(c, L.total);
+  if (rc) return rc;
+  tcd::Prep p = make_prep(c, L, false);
+  record(c, 0, s);
+  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_fill_i32(p.vmark, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_fill_i32(p.vcnt, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_mark_list(d_list, nlist, nv, p.vmark, s)) != hipSuccess) return hip_fail(c, e, "mark");
+  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
+  if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 2, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "edge list");
+  record(c, 1, s);
+  rc = run_count(c, p, cfg, vi, d_ind, p.vcnt, nullptr, 1, s, true);
+  if (rc) return rc;
+  c->last_dscal = p.dscal;
+  record(c, 2, s);
+  if ((e = tcd::launch_gather_list(d_list, nlist, nv, p.vcnt, d_out, d_total, s)) != hipSuccess)
+    return hip_fail(c, e, "gather");
   record(c, 3, s);
   c->ev_recorded = c->timing;
   return TC_OK;

@@ -28,6 +28,7 @@ struct CountArgs {
   unsigned int* qctr;           // 8 per-XCD range queue counters, zeroed per count
   int32_t wave_acap;            // streamed kernel: longest shorter-list a single wave stages
   int32_t search_ratio;         // edges with |B| >= search_ratio * |A| go to k_count_search (0: none)
+  int32_t u_only;               // credit each edge's count to out[eu] only (vertex-list mode)
 };
 
 // Kernel variants compiled into the library.
@@ -41,6 +42,8 @@ Variant variant(int i);
 struct Prep {
   int64_t* voff;    // [nv + 1]
   int32_t* vbase;   // [nv] graph v_base per vertex, or nullptr (single graph)
+  int32_t* vmark;   // [nv] vertex-list mode: 1 for listed vertices, else nullptr
+  int32_t* vcnt;    // [nv] vertex-list mode: per-vertex counts before the gather
   int32_t* ecnt;    // [nv]
   int64_t* eoff;    // [nv + 1]
   int32_t* eu;      // [ecap]
@@ -77,6 +80,9 @@ int resident_wgs2(int variant_idx, int cus, int which);
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
 int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
+hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
+hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
+                              int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,
                            int cus, hipStream_t s);
 hipError_t launch_mismatch(const int32_t* a, const int32_t* b, int64_t n, int32_t* cnt, hipStream_t s);

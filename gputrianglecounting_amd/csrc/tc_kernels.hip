@@ -191,10 +191,15 @@ hipError_t scan_excl(const Tin* in, const int64_t* n_dev, int64_t n_cap, int64_t
 // ---------------------------------------------------------------- edge list
 // One wave per vertex.  mode 0: count neighbours w > v (graph-local ids)
 // with d(v) >= 2 and d(w) >= 2.  mode 1 (oriented CSR): count neighbours w
-// with d+(v) >= 1 and d+(w) >= 1.
+// with d+(v) >= 1 and d+(w) >= 1.  mode 2 (vertex list): every arc v -> w of
+// a marked v with w != v, d(v) >= 2 and d(w) >= 2 (the reference's per-arc
+// rule, clusteringCount.cu:295-298).
 __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
-  return mode == 0 ? (wl > vl && dv >= 2 && dw >= 2) : (dv >= 1 && dw >= 1);
+  if (mode == 0) return wl > vl && dv >= 2 && dw >= 2;
+  if (mode == 2) return wl != vl && dv >= 2 && dw >= 2;
+  return dv >= 1 && dw >= 1;
 }
+__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 ? 1 : 2; }
 
 // Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
 // so hub rows do not pile up on one wave; each lane keeps 4 arc loads in
@@ -209,7 +214,8 @@ __device__ __forceinline__ int64_t grab_vertices(unsigned long long* ctr) {
 
 __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
                                                     const int32_t* __restrict__ vbase, int64_t nv, int mode,
-                                                    int32_t* __restrict__ ecnt, unsigned long long* ctr) {
+                                                    const int32_t* __restrict__ vmark, int32_t* __restrict__ ecnt,
+                                                    unsigned long long* ctr) {
   const int lane = threadIdx.x & 63;
   for (;;) {
     const int64_t v0 = grab_vertices(ctr);
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ 
       const int32_t vb = vbase ? vbase[v] : 0;
       const int32_t vl = (int32_t)(v - vb);
       int32_t c = 0;
-      if (dv >= (mode == 0 ? 2 : 1)) {
+      if (dv >= edge_min_deg(mode) && (mode != 2 || vmark[v])) {
         for (int64_t k0 = b; k0 < e; k0 += 256) {
           int32_t wl[4];
 #pragma unroll
@@ -247,7 +253,8 @@ __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ 
 
 __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
                                                    const int32_t* __restrict__ vbase, int64_t nv, int mode,
-                                                   const int64_t* __restrict__ eoff, int64_t ecap,
+                                                   const int32_t* __restrict__ vmark, const int64_t* __restrict__ eoff,
+                                                   int64_t ecap,
                                                    int32_t* __restrict__ eu, int32_t* __restrict__ ew,
                                                    int32_t* __restrict__ ework, unsigned long long* ctr) {
   const int lane = threadIdx.x & 63;
@@ -258,7 +265,7 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
     const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
     for (int64_t v = v0; v < v1; ++v) {
       const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-      if (dv < (mode == 0 ? 2 : 1)) continue;
+      if (dv < edge_min_deg(mode) || (mode == 2 && !vmark[v])) continue;
       const int32_t vb = vbase ? vbase[v] : 0;
       const int32_t vl = (int32_t)(v - vb);
       int64_t pos = eoff[v];
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(256) void k_count_simple(CountArgs a) {
     if (c) {
       if (a.out) {
         atomicAdd(&a.out[u], c);
-        atomicAdd(&a.out[w], c);
+        if (!a.u_only) atomicAdd(&a.out[w], c);
       }
       my += c;
     }
@@ -608,6 +615,32 @@ unsigned grid_for(int64_t n, int bs, int64_t cap = 65536) {
   return (unsigned)g;
 }
 
+// ---------------------------------------------------------------- vertex list
+// vmark[v] = 1 for every listed v in [0, nv) (entries outside are skipped).
+__global__ void k_mark_list(const int32_t* __restrict__ list, int64_t nlist, int64_t nv, int32_t* __restrict__ vmark) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nlist; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = list[i];
+    if (v >= 0 && v < nv) vmark[v] = 1;
+  }
+}
+
+// out[i] = vcnt[list[i]] (-1 for an entry outside [0, nv)), and the total
+// of the listed counts added to *total.
+__global__ __launch_bounds__(256) void k_gather_list(const int32_t* __restrict__ list, int64_t nlist, int64_t nv,
+                                                     const int32_t* __restrict__ vcnt, int32_t* __restrict__ out,
+                                                     unsigned long long* total) {
+  __shared__ int64_t red[4];
+  int64_t my = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nlist; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = list[i];
+    const int32_t c = (v >= 0 && v < nv) ? vcnt[v] : -1;
+    out[i] = c;
+    my += c > 0 ? c : 0;
+  }
+  const int64_t s = block_sum64<256>(my, red);
+  if (threadIdx.x == 0 && total && s) atomicAdd(total, (unsigned long long)s);
+}
+
 // Compiled count kernels: algo 0 = streamed (k_count_bs) with every long
 // edge on one wave (A in windows), launched as that wave kernel fn then a
 // small-tile kernel fn2; 3 = streamed with the longest edges left to a
@@ -696,17 +729,30 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
   hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
-    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode, p.ecnt,
-                       ctr);
+    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
+                       (const int32_t*)p.vmark, p.ecnt, ctr);
   e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.eoff, p.scan_tmp, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
-                       (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework, ctr + 1);
+                       (const int32_t*)p.vmark, (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework, ctr + 1);
   hipLaunchKernelGGL(k_edge_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.eoff, nv, p.ecap, p.dscal);
   e = scan_excl<int32_t>(p.ework, p.dscal, p.ecap, p.wpre, p.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_work_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.wpre, p.dscal);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {
+  if (nlist <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_list, dim3(grid_for(nlist, 256, 16384)), dim3(256), 0, s, list, nlist, nv, vmark);
+  return hipGetLastError();
+}
+hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
+                              int64_t* total, hipStream_t s) {
+  if (nlist <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_list, dim3(grid_for(nlist, 256, 4096)), dim3(256), 0, s, list, nlist, nv, vcnt, out,
+                     reinterpret_cast<unsigned long long*>(total));
   return hipGetLastError();
 }
 

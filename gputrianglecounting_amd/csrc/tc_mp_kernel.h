@@ -217,7 +217,7 @@ __global__ __launch_bounds__(BS) void k_count_mp(CountArgs a) {
       if (tid == 0 && c) {
         if (a.out) {
           atomicAdd(&a.out[u], (int32_t)c);
-          atomicAdd(&a.out[w], (int32_t)c);
+          if (!a.u_only) atomicAdd(&a.out[w], (int32_t)c);
         }
         my_sum += c;
       }
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(BS) void k_count_mp(CountArgs a) {
       if (tid < c) {
         const int32_t cc = m_cnt[tid];
         if (cc) {
-          if (a.out) atomicAdd(&a.out[a.ew[ee]], cc);
+          if (a.out && !a.u_only) atomicAdd(&a.out[a.ew[ee]], cc);
           my_sum += cc;
         }
         // u side: the first edge of each run of equal u adds the run's sum

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(BS) void k_count_search(CountArgs a) {
         if (cc) {
           if (a.out) {
             atomicAdd(&a.out[s_u[tid]], cc);
-            atomicAdd(&a.out[s_w[tid]], cc);
+            if (!a.u_only) atomicAdd(&a.out[s_w[tid]], cc);
           }
           my_sum += cc;
         }

@@ -69,6 +69,9 @@ def host_lib() -> C.CDLL:
         "tch_batch_kind": (_I32P, [vp]),
         "tch_batch_graph": (vp, [vp, C.c_int32]),
         "tch_batch_free": (None, [vp]),
+        "tch_read_partition": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_I32P)]),
+        "tch_partition_members": (C.c_int, [C.c_int32, _I32P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_I32P)]),
+        "tch_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -200,6 +203,36 @@ def read_info(path_no_ext: str) -> Graph:
 def read_metis(path: str) -> Graph:
     """METIS/DIMACS10 adjacency reader (csrdatacreate.cpp:3-49); rows sorted, nothing dropped."""
     return _from_handle(host_lib().tch_read_metis(path.encode()), os.path.basename(path))
+
+
+def _take_i32(ptr, n: int) -> np.ndarray:
+    L = host_lib()
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+    finally:
+        L.tch_free(C.cast(ptr, C.c_void_p))
+
+
+def read_partition(path_no_ext: str, k: int) -> np.ndarray:
+    """The integers of ``<path_no_ext>.part.<k>`` after its count header
+    (graphRead.cpp:28-45)."""
+    n = C.c_int32(0)
+    ptr = _I32P()
+    rc = host_lib().tch_read_partition(path_no_ext.encode(), k, C.byref(n), C.byref(ptr))
+    if rc != 0:
+        raise HostError(_err())
+    return _take_i32(ptr, n.value)
+
+
+def partition_members(part, p: int) -> np.ndarray:
+    """Vertices v with part[v] == p, ascending (part: a METIS partition vector)."""
+    part = np.ascontiguousarray(part, np.int32)
+    cnt = C.c_int32(0)
+    ptr = _I32P()
+    rc = host_lib().tch_partition_members(part.shape[0], _ptr32(part), p, C.byref(cnt), C.byref(ptr))
+    if rc != 0:
+        raise HostError(_err())
+    return _take_i32(ptr, cnt.value)
 
 
 def read_bin(path: str) -> Graph:

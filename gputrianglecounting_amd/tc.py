@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         "tc_validate_i32": (C.c_int, [vp, i32, i64, vp, vp, C.POINTER(C.c_int), vp]),
         "tc_batch_count_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_count_total_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "tc_count_vertices_i32": (C.c_int, [vp, i32, i64, vp, vp, i32, vp, vp, vp, vp, vp]),
         "tc_device_cus": (C.c_int, [vp]),
         "tc_last_edge_work": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "tc_copy_i32": (C.c_int, [vp, vp, vp, i64, vp]),
@@ -130,7 +131,7 @@ def lib() -> C.CDLL:
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
            "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
-           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
+           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
            "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
            "tc_server_submit", "tc_server_wait", "tc_server_get_stats"]
@@ -261,6 +262,26 @@ class TriangleCounter:
                                       _stream(stream))
         self._check(rc, "tc_count_total_i32")
         return total
+
+    def count_vertices(self, off, ind, vertices, out=None, total=None, cfg=None, stream=None):
+        """Vertex-list count (the reference's vertex-list kernelCall overload):
+        ``out[i]`` = the per-vertex count of ``vertices[i]`` (-1 for an id
+        outside the graph), ``total`` int64[1] = their sum."""
+        torch = self._torch
+        nv = int(off.shape[0]) - 1
+        nnz = int(ind.shape[0])
+        nl = int(vertices.shape[0])
+        if vertices.dtype != torch.int32:
+            raise TypeError("vertices must be int32")
+        if out is None:
+            out = torch.empty(max(nl, 0), dtype=torch.int32, device=off.device)
+        if total is None:
+            total = torch.empty(1, dtype=torch.int64, device=off.device)
+        rc = lib().tc_count_vertices_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, nl,
+                                         _p(vertices) if nl else None, _p(out) if nl else None, _p(total),
+                                         _cfgp(cfg), _stream(stream))
+        self._check(rc, "tc_count_vertices_i32")
+        return out, total
 
     def lcc(self, off, out, lcc=None, stream=None):
         torch = self._torch

@@ -122,6 +122,21 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
                       const int32_t* d_ind, int32_t* outPutTriangles, int32_t threads_per_block,
                       int32_t number_blocks, int32_t shifter);
 
+/* Vertex-list count: the reference's vertex-list overload of kernelCall<T>
+ * (src/clusteringCount.cu:361-413, kernel :266-312; CPU form
+ * src/clusteringCount.cpp:125-155).  d_list[nlist] holds vertex ids of the
+ * graph (any order, repeats allowed); d_out[i] receives, for v = d_list[i],
+ * the sum over arcs v -> w with w != v, d(v) >= 2 and d(w) >= 2 of
+ * |N(v) ∩ N(w)| (2·T(v) on a simple graph), and d_total (nullable) the sum
+ * of d_out.  An entry outside [0, nv) gets d_out[i] = -1 and adds nothing.
+ * The reference overload takes neither the graph's vertex count nor its arc
+ * count; this entry point needs both (nv, nnz) for its scratch.  Partition
+ * lists of a .part.K file (tch_read_partition in tc_host.h) are the
+ * intended input. */
+int tc_count_vertices_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                          int32_t nlist, const int32_t* d_list, int32_t* d_out, int64_t* d_total, const tc_cfg* cfg,
+                          void* stream);
+
 /* FP64 local clustering coefficient lcc[v] = out[v] / (d(v)·(d(v)-1)),
  * 0 when d(v) < 2; one IEEE division of exactly converted integers. */
 int tc_lcc_f64(tc_ctx* ctx, int32_t nv, const int32_t* d_off, const int32_t* d_out, double* d_lcc,

@@ -78,6 +78,16 @@ tch_graph* tch_read_info(const char* path_no_ext);
 int        tch_write_info(const tch_graph* g, const char* path);
 tch_graph* tch_read_metis(const char* path);
 
+/* Partition file (graphRead.cpp:28-45): path_no_ext + ".part.K" holds a
+ * count n and then n integers, whitespace separated.  *vals receives them
+ * (malloc'd; free with tch_free) and *n their number.  With a METIS
+ * partition vector (one partition id per vertex, the count first) the
+ * members of partition p are tch_partition_members(n, vals, p, ...): the
+ * vertices v with part[v] == p in ascending order (malloc'd). */
+int  tch_read_partition(const char* path_no_ext, int32_t K, int32_t* n, int32_t** vals);
+int  tch_partition_members(int32_t n, const int32_t* part, int32_t p, int32_t* count, int32_t** ids);
+void tch_free(void* p);
+
 /* Binary CSR cache: a 32-byte header ("TCCSR001", n, nnz, reserved) then
  * off and ind as little-endian int32. */
 int        tch_write_bin(const tch_graph* g, const char* path);

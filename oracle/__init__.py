@@ -146,6 +146,27 @@ def ref_count_vertex_list(vertices, off, ind):
     return tri[:narcs], int(tot[0])
 
 
+def count_vertex_list(vertices, off, ind):
+    """The C restatement of the vertex-list overload (clusteringCount.cpp:125-155):
+    per-arc counts of the listed rows in list order, and their total."""
+    off, ind, vl = _c(off, np.int32), _c(ind, np.int32), _c(vertices, np.int32)
+    narcs = int(sum(int(off[v + 1]) - int(off[v]) for v in vl))
+    tri = np.zeros(max(narcs, 1), np.int64)
+    tot = np.zeros(1, np.int64)
+    lib().oracle_count_vertex_list(vl.shape[0], _p32(vl), _p32(off), _p32(ind), _p64(tri), _p64(tot))
+    return tri[:narcs], int(tot[0])
+
+
+def list_sums(vertices, off, tri):
+    """Per-list-position sums of vertex-list per-arc counts (the value the
+    reference's vertex-list kernel stores at outPutTriangles[i])."""
+    off = np.asarray(off, np.int64)
+    lens = np.array([off[v + 1] - off[v] for v in vertices], np.int64)
+    cs = np.concatenate([[0], np.cumsum(tri, dtype=np.int64)])
+    ends = np.cumsum(lens)
+    return cs[ends] - cs[ends - lens]
+
+
 def row_sums(off, tri):
     off = np.asarray(off, np.int64)
     cs = np.concatenate([[0], np.cumsum(tri, dtype=np.int64)])

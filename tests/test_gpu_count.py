@@ -84,6 +84,55 @@ def test_search_ratio(counter, algo, ratio, ranges):
         _check(counter, g, cfg=dict(algo=algo, search_ratio=ratio, ranges_per_wg=ranges))
 
 
+def _list_expect(g, vl):
+    """Oracle vertex-list values; -1 for ids outside the graph."""
+    vl = np.asarray(vl, np.int32)
+    ok = (vl >= 0) & (vl < g.n)
+    tri, _ = oracle.count_vertex_list(vl[ok], g.off, g.ind)
+    exp = np.full(vl.shape[0], -1, np.int64)
+    exp[ok] = oracle.list_sums(vl[ok], g.off, tri)
+    return exp, int(exp[ok].sum())
+
+
+@pytest.mark.parametrize("cfg", [None, dict(search_ratio=1), dict(search_ratio=-1), dict(algo=3), dict(algo=2),
+                                 dict(algo=1), dict(small_max=2048)], ids=str)
+def test_vertex_list(counter, cfg):
+    import torch
+
+    g = G.rmat(13, 32, seed=3)
+    off, ind = counter.to_device(g)
+    rng = np.random.default_rng(5)
+    lists = [np.array([0, 1, 2, 0, 4095, 8191, 17], np.int32),
+             rng.integers(0, g.n, 3000).astype(np.int32),
+             np.arange(g.n, dtype=np.int32)[::-1].copy(),
+             np.array([-3, 5, g.n, 5, 2**31 - 1], np.int32),
+             np.zeros(0, np.int32)]
+    for vl in lists:
+        print(f"case vertex list len={vl.shape[0]} cfg={cfg}", flush=True)
+        exp, etot = _list_expect(g, vl)
+        out, tot = counter.count_vertices(off, ind, torch.from_numpy(vl).to(off.device), cfg=cfg)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy().astype(np.int64), exp)
+        assert int(tot.item()) == etot
+
+
+def test_partitions_sum_to_the_whole(counter):
+    import torch
+
+    g = G.rmat(14, 16, seed=8)
+    off, ind = counter.to_device(g)
+    full, ftot = counter.count_graph(g)
+    part = (np.arange(g.n) * 2654435761 % 5).astype(np.int32)  # 5 partitions
+    tot = 0
+    for p in range(5):
+        vl = G.partition_members(part, p)
+        out, t = counter.count_vertices(off, ind, torch.from_numpy(vl).to(off.device))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), full[vl])
+        tot += int(t.item())
+    assert tot == ftot
+
+
 def test_simple_kernel_agrees(counter):
     g = G.rmat(12, 16, seed=4)
     _check(counter, g, dict(algo=1))
@@ -190,3 +239,26 @@ def test_errors(counter):
         counter.count(off, ind, cfg=dict(wg_size=100))
     with pytest.raises(T.TCError):
         counter.count(off, ind, cfg=dict(ranges_per_wg=100000))
+
+
+def test_tc_main_partition_run(tmp_path):
+    """tc_main <graph> 1 <blocks> <bs> <tpi> <K>: the partitions of
+    <graph>.part.K counted as vertex lists sum to the whole-graph total."""
+    import os
+    import subprocess
+
+    g = G.rmat(12, 16, seed=6)
+    base = str(tmp_path / "g")
+    g.write_info(base + ".info")
+    part = (np.arange(g.n) % 3).astype(np.int32)
+    with open(base + ".part.3", "w") as f:
+        f.write(f"{g.n}\n" + "\n".join(str(int(x)) for x in part) + "\n")
+    exe = os.path.join(os.path.dirname(T.__file__), "_lib", "tc_main")
+    print("case tc_main partition run", flush=True)
+    r = subprocess.run([exe, base, "1", "8", "256", "4", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    nv, total = r.stdout.split()[:2]
+    _, rtot = _ref(g)
+    assert int(nv) == g.n and int(total) == rtot
+    r = subprocess.run([exe, base, "1", "8", "256", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and int(r.stdout.split()[1]) == rtot

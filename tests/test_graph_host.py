@@ -234,3 +234,23 @@ def test_pack_round_trip():
         np.testing.assert_array_equal(h.off, g.off)
         np.testing.assert_array_equal(h.ind, g.ind)
     assert b.total_arcs == sum(g.nnz for g in gs)
+
+
+# ---------------------------------------------------------------- partitions
+def test_partition_file(tmp_path):
+    """.part.K files as graphRead.cpp:28-45 reads them: a count, then that
+    many integers; members of one partition from a METIS partition vector."""
+    base = str(tmp_path / "g")
+    with open(base + ".part.3", "w") as f:
+        f.write("7\n0\n2\n1\n0\n2 2\n1\n")
+    part = G.read_partition(base, 3)
+    assert part.tolist() == [0, 2, 1, 0, 2, 2, 1]
+    assert G.partition_members(part, 0).tolist() == [0, 3]
+    assert G.partition_members(part, 2).tolist() == [1, 4, 5]
+    assert G.partition_members(part, 9).tolist() == []
+    with pytest.raises(G.HostError):
+        G.read_partition(base, 4)  # no .part.4
+    with open(base + ".part.5", "w") as f:
+        f.write("4\n0 1 2\n")
+    with pytest.raises(G.HostError):
+        G.read_partition(base, 5)  # truncated

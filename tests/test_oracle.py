@@ -107,6 +107,22 @@ def test_reference_vertex_list_agrees():
     assert tot == int(expect.sum())
 
 
+def test_vertex_list_restatement():
+    """The C restatement of the vertex-list overload against the reference
+    build (when present) and, on a simple graph, against per-vertex counts
+    gathered at the listed ids (repeats included)."""
+    g = G.rmat(11, 16, seed=2)
+    vl = np.array([5, 0, 77, 1500, 3, 0, 5], np.int32)
+    tri, tot = oracle.count_vertex_list(vl, g.off, g.ind)
+    if oracle.ref_lib() is not None:
+        rtri, rtot = oracle.ref_count_vertex_list(vl, g.off, g.ind)
+        np.testing.assert_array_equal(tri, rtri)
+        assert tot == rtot
+    pv, _ = oracle.per_vertex(g.off, g.ind)
+    np.testing.assert_array_equal(oracle.list_sums(vl, g.off, tri), pv[vl].astype(np.int64))
+    assert tot == int(pv[vl].astype(np.int64).sum())
+
+
 # ---------------------------------------------------------------- closed forms
 @pytest.mark.parametrize("n", [3, 4, 5, 12, 40])
 def test_complete(n):
