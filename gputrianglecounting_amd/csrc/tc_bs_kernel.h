@@ -179,11 +179,15 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         };
-        // B elements at ind4 positions [j0, j1) against the staged A
+        // B elements at ind4 positions [j0, j1) against the staged A; an
+        // edge with B at least probe_ratio times longer than A is walked
+        // from A's side (wave-uniform choice)
+        const bool by_a = a.probe_ratio > 0 && (int64_t)lB >= (int64_t)a.probe_ratio * lA;
         auto stream = [&](int64_t j0, int64_t j1, int32_t n) -> int32_t {
           const int64_t cb = j0 / CH, nc = (j1 + CH - 1) / CH - cb;
           const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
-          return stream_chunks<CH, SD>(ind4, c_lo, c_hi, j0, j1, abuf, n);
+          return by_a ? stream_chunks_a<CH, SD>(ind4, c_lo, c_hi, j0, j1, abuf, n)
+                      : stream_chunks<CH, SD>(ind4, c_lo, c_hi, j0, j1, abuf, n);
         };
         const int64_t g0 = gB + sh + (t0 * lB) / L, g1 = gB + sh + (t1 * lB) / L;
         int32_t c = 0;

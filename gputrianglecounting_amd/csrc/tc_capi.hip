@@ -51,6 +51,7 @@ tc_cfg builtin_default() {
   c.depth = 2;
   c.algo = 0;
   c.search_ratio = 64;
+  c.probe_ratio = 4;
   return c;
 }
 
@@ -67,6 +68,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->chunk) c.chunk = in->chunk;
     if (in->depth) c.depth = in->depth;
     if (in->search_ratio) c.search_ratio = in->search_ratio;
+    if (in->probe_ratio) c.probe_ratio = in->probe_ratio;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -198,6 +200,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.qctr = p.qctr;
   a.search_ratio = (cfg.algo == 0 || cfg.algo == 3) && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
   a.u_only = u_only ? 1 : 0;
+  a.probe_ratio = cfg.probe_ratio > 0 ? cfg.probe_ratio : 0;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
   int nwg3 = tcd::resident_search_wgs(c->cus);

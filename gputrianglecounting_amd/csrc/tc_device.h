@@ -29,6 +29,7 @@ struct CountArgs {
   int32_t wave_acap;            // streamed kernel: longest shorter-list a single wave stages
   int32_t search_ratio;         // edges with |B| >= search_ratio * |A| go to k_count_search (0: none)
   int32_t u_only;               // credit each edge's count to out[eu] only (vertex-list mode)
+  int32_t probe_ratio;          // wave phase: edges with |B| >= probe_ratio * |A| walk A per chunk (0: never)
 };
 
 // Kernel variants compiled into the library.

@@ -494,6 +494,82 @@ __device__ __forceinline__ bool edge_search(int32_t lA, int32_t lB, int32_t rati
   return ratio > 0 && lB >= kSearchMinB && (int64_t)lB >= (int64_t)ratio * lA;
 }
 
+// The same count as stream_chunks, driven by A instead of B: for each chunk
+// the lane walks the A values up to the chunk's largest element (each A
+// value is visited once over the lane's run, the next one already loading)
+// and tests each against the chunk's CH elements held in registers.  Costs
+// about CH compares per A value and no branch per B element, so it wins
+// when A is much shorter than B (few A values per chunk).  Needs
+// A[lA] = INT_MAX (a value above every chunk element).
+template <int CH, int D>
+__device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
+                                                   int64_t g0, int64_t g1, const int32_t* A, int32_t lA) {
+  constexpr int Q = CH / 4;
+  if (c_lo >= c_hi) return 0;
+  const int4* b4 = reinterpret_cast<const int4*>(base);
+  int4 slot[D][Q];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (c_lo + k < c_hi) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) slot[k][q] = b4[(c_lo + k) * Q + q];
+    }
+  const int64_t gf = c_lo * CH > g0 ? c_lo * CH : g0;
+  const int32_t bf = base[gf];
+  int32_t lo = 0, hi = lA;  // lower bound of bf in A
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (A[mid] < bf) lo = mid + 1;
+    else hi = mid;
+  }
+  int32_t pos = lo;
+  int32_t acur = A[pos];
+  int32_t cnt = 0;
+  for (int64_t c0 = c_lo; c0 < c_hi; c0 += D) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int64_t c = c0 + k;
+      if (c < c_hi) {
+        int32_t b[CH];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          b[4 * q] = slot[k][q].x;
+          b[4 * q + 1] = slot[k][q].y;
+          b[4 * q + 2] = slot[k][q].z;
+          b[4 * q + 3] = slot[k][q].w;
+        }
+        if (c + D < c_hi) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) slot[k][q] = b4[(c + D) * Q + q];
+        }
+        const int64_t cbase = c * CH;
+        if (cbase < g0 || cbase + CH > g1) {
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int64_t g = cbase + j;
+            if (g < g0 || g >= g1) b[j] = -1;  // ids are >= 0: matches nothing
+          }
+        }
+        int32_t bmax = b[0];
+#pragma unroll
+        for (int j = 1; j < CH; ++j) bmax = b[j] > bmax ? b[j] : bmax;
+        // A values in (previous chunk's max, bmax]; acur <= bmax < INT_MAX
+        // keeps pos + 1 <= lA, so the prefetch stays on the staged list
+        while (acur <= bmax) {
+          const int32_t anext = A[pos + 1];
+          bool hit = false;
+#pragma unroll
+          for (int j = 0; j < CH; ++j) hit |= b[j] == acur;
+          cnt += hit;
+          ++pos;
+          acur = anext;
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
 #include "tc_bs_kernel.h"
 #include "tc_search_kernel.h"
 

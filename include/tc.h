@@ -58,7 +58,11 @@ typedef struct tc_ctx tc_ctx;
  *                  at least search_ratio times as many elements as the
  *                  shorter list A is counted by binary-searching every
  *                  element of A in B instead of streaming B (default 64;
- *                  negative: never) */
+ *                  negative: never)
+ *   probe_ratio    algo 0/3: a streamed edge whose B is at least probe_ratio
+ *                  times longer than A is merged from A's side, each A value
+ *                  tested against a 16-element chunk of B in registers
+ *                  (default 4; negative: never) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
@@ -69,6 +73,7 @@ typedef struct tc_cfg {
   int32_t chunk;
   int32_t depth;
   int32_t search_ratio;
+  int32_t probe_ratio;
 } tc_cfg;
 
 enum {

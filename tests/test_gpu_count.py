@@ -84,6 +84,16 @@ def test_search_ratio(counter, algo, ratio, ranges):
         _check(counter, g, cfg=dict(algo=algo, search_ratio=ratio, ranges_per_wg=ranges))
 
 
+@pytest.mark.parametrize("algo", [0, 3])
+@pytest.mark.parametrize("probe", [1, 4, -1])
+@pytest.mark.parametrize("search", [-1, 64])
+def test_probe_ratio(counter, algo, probe, search):
+    """Streamed edges with B at least probe times longer than A are merged
+    from A's side; probe 1 sends every streamed edge there, -1 none."""
+    for g in _search_graphs():
+        _check(counter, g, cfg=dict(algo=algo, probe_ratio=probe, search_ratio=search, ranges_per_wg=4096))
+
+
 def _list_expect(g, vl):
     """Oracle vertex-list values; -1 for ids outside the graph."""
     vl = np.asarray(vl, np.int32)
