@@ -44,7 +44,7 @@
 // LDS budget of its own code; both launches walk the same ranges and split
 // the edges by the same rule (a.wave_acap = the phase-1 A capacity AWT).
 template <int BS, int CH, int P, int M, int K, int SD, int PH, int AWT>
-__global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu((PH & 1) ? 4 : 1))) void k_count_bs(CountArgs a) {
   constexpr int NW = BS / 64;
   constexpr int AW = AWT;                // per-wave A capacity
   constexpr int BUFW = (PH & 6) ? 2 * P : NW * (AW + 1);
@@ -144,14 +144,45 @@ __global__ __launch_bounds__(BS) void k_count_bs(CountArgs a) {
       const int wv = __builtin_amdgcn_readfirstlane(wid);
       int32_t* abuf = buf + wv * (AW + 1);
       int32_t cnt_w = 0;
-      for (int64_t e = e_first + wv; e <= e_last; e += NW) {
-        const int64_t we0 = a.wpre[e], L = a.wpre[e + 1] - we0;
+      // Edge records are fetched two edges ahead of use: lanes 0-5 each load
+      // one 32-bit word of (eu, ew, wpre[e], wpre[e+1]), lanes 0-7 one word
+      // of the row offsets; readlane makes them uniform.  Vector loads, so
+      // the merge's LDS waits do not wait on them.
+      auto rec_load = [&](int64_t e) -> int32_t {
+        if (e > e_last || lane >= 6) return 0;
+        const int32_t* src = lane == 0 ? a.eu + e : lane == 1 ? a.ew + e
+                                                 : reinterpret_cast<const int32_t*>(a.wpre + e) + (lane - 2);
+        return *src;
+      };
+      // row offsets as 32-bit words: lane 2i loads the low and lane 2i+1 the
+      // high word of offset i (voff[u], voff[u+1], voff[w], voff[w+1])
+      auto off_load = [&](int32_t r, bool valid) -> int32_t {
+        const int32_t u = __builtin_amdgcn_readlane(r, 0), w = __builtin_amdgcn_readlane(r, 1);
+        if (!valid || lane >= 8) return 0;
+        const int64_t* src = a.voff + (lane < 4 ? u : w) + ((lane >> 1) & 1);
+        return reinterpret_cast<const int32_t*>(src)[lane & 1];
+      };
+      auto lo_hi = [](int32_t r, int l) -> int64_t {
+        return (int64_t)(uint32_t)__builtin_amdgcn_readlane(r, l) | ((int64_t)__builtin_amdgcn_readlane(r, l + 1) << 32);
+      };
+      int64_t e = e_first + wv;
+      int32_t rec = rec_load(e);
+      int32_t rec_n = rec_load(e + NW);
+      int32_t offs = off_load(rec, e <= e_last);
+      for (; e <= e_last; e += NW) {
+        const int32_t u = __builtin_amdgcn_readlane(rec, 0), w = __builtin_amdgcn_readlane(rec, 1);
+        const int64_t we0 = lo_hi(rec, 2), L = lo_hi(rec, 4) - we0;
+        const int64_t ab = lo_hi(offs, 0), ae = lo_hi(offs, 2), bb = lo_hi(offs, 4), be = lo_hi(offs, 6);
+        // start the next two edges' fetches before this edge's work
+        const int32_t rec_nn = rec_load(e + 2 * NW);
+        const int32_t offs_n = off_load(rec_n, e + NW <= e_last);
+        rec = rec_n;
+        rec_n = rec_nn;
+        offs = offs_n;
         const int64_t t0 = R0 > we0 ? R0 - we0 : 0;
         const int64_t t1 = R1 - we0 < L ? R1 - we0 : L;
         if (t0 == 0 && t1 == L && L <= a.small_max) continue;  // phase 3
-        const int32_t u = a.eu[e], w = a.ew[e];
-        const int64_t ab = a.voff[u], bb = a.voff[w];
-        const int32_t la = (int32_t)(a.voff[u + 1] - ab), lb = (int32_t)(a.voff[w + 1] - bb);
+        const int32_t la = (int32_t)(ae - ab), lb = (int32_t)(be - bb);
         const bool a_is_u = la <= lb;
         const int32_t lA = a_is_u ? la : lb, lB = a_is_u ? lb : la;
         if (edge_search(lA, lB, a.search_ratio)) continue;  // k_count_search
