@@ -52,6 +52,7 @@ tc_cfg builtin_default() {
   c.algo = 0;
   c.search_ratio = 64;
   c.probe_ratio = 4;
+  c.oriented = 1;
   return c;
 }
 
@@ -69,6 +70,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->depth) c.depth = in->depth;
     if (in->search_ratio) c.search_ratio = in->search_ratio;
     if (in->probe_ratio) c.probe_ratio = in->probe_ratio;
+    if (in->oriented) c.oriented = in->oriented > 0 ? 1 : 0;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -164,6 +166,7 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.ecap = L.ecap;
   p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
   p.oind = L.ocap ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
+  p.ocap = L.ocap;
   p.vmark = L.list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
   p.vcnt = L.list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
@@ -335,7 +338,8 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv, nnz, false, false);
+  if (cfg.algo != 0) cfg.oriented = 0;  // the oriented kernel is algo 0's
+  const Layout L = layout(nv, nnz, false, cfg.oriented != 0);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, false);
@@ -343,10 +347,23 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-  if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
-    return hip_fail(c, e, "edge list");
-  record(c, 1, s);
-  rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
+  if (cfg.oriented) {
+    // Degree-ordered orientation: triangle {a, b, c} with (d, id) order
+    // a < b < c is found once, at arc a -> b with match c; every one of the
+    // three gets 2 (out[v] = 2 T(v)), the total 6 per triangle.
+    // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
+    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    record(c, 1, s);
+    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.probe_ratio > 0 ? cfg.probe_ratio : 0, c->cus, s)) !=
+        hipSuccess)
+      return hip_fail(c, e, "count kernel launch");
+  } else {
+    if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "edge list");
+    record(c, 1, s);
+    rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
+  }
   if (rc) return rc;
   c->last_dscal = p.dscal;
   record(c, 2, s);

@@ -57,7 +57,8 @@ struct Prep {
   int64_t ecap;
   // oriented copy for the total-only mode
   int64_t* ooff;    // [nv + 1]
-  int32_t* oind;    // [nnz / 2 + 1]
+  int32_t* oind;    // [ocap]
+  int64_t ocap;
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -82,6 +83,9 @@ hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, 
 int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
+// Per-vertex counts over the oriented CSR in p.ooff / p.oind (k_count_orient).
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio, int cus,
+                               hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

@@ -194,12 +194,18 @@ hipError_t scan_excl(const Tin* in, const int64_t* n_dev, int64_t n_cap, int64_t
 // with d+(v) >= 1 and d+(w) >= 1.  mode 2 (vertex list): every arc v -> w of
 // a marked v with w != v, d(v) >= 2 and d(w) >= 2 (the reference's per-arc
 // rule, clusteringCount.cu:295-298).
+__device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, int64_t w) {
+  return dv < dw || (dv == dw && v < w);
+}
+// mode 3 (orientation): keep arc v -> w iff (d(v), v) < (d(w), w); the
+// fill then writes only w, building the oriented CSR row by row.
 __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
   if (mode == 0) return wl > vl && dv >= 2 && dw >= 2;
   if (mode == 2) return wl != vl && dv >= 2 && dw >= 2;
+  if (mode == 3) return orient_keep(dv, vl, dw, wl);
   return dv >= 1 && dw >= 1;
 }
-__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 ? 1 : 2; }
+__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 || mode == 3 ? 1 : 2; }
 
 // Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
 // so hub rows do not pile up on one wave; each lane keeps 4 arc loads in
@@ -291,9 +297,11 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
           if (keep) {
             const int64_t idx = pos + __popcll(m & lt);
             if (idx < ecap) {
-              eu[idx] = (int32_t)v;
               ew[idx] = (int32_t)((int64_t)vb + wl[r]);
-              ework[idx] = (int32_t)(dv + dw);
+              if (mode != 3) {
+                eu[idx] = (int32_t)v;
+                ework[idx] = (int32_t)(dv + dw);
+              }
             }
           }
           pos += __popcll(m);
@@ -312,52 +320,6 @@ __global__ void k_edge_finish(const int64_t* __restrict__ eoff, int64_t nv, int6
 }
 __global__ void k_work_finish(const int64_t* __restrict__ wpre, int64_t* __restrict__ dscal) {
   if (blockIdx.x == 0 && threadIdx.x == 0) dscal[1] = wpre[dscal[0]];
-}
-
-// Oriented copy for the total-only mode: keep arc v -> w iff
-// (d(v), v) < (d(w), w); rows stay sorted.
-__device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, int64_t w) {
-  return dv < dw || (dv == dw && v < w);
-}
-__global__ __launch_bounds__(256) void k_orient_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
-                                                      int64_t nv, int32_t* __restrict__ ocnt) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t v = wave; v < nv; v += nwaves) {
-    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-    int32_t c = 0;
-    for (int64_t k = b + lane; k < e; k += 64) {
-      const int32_t w = ind[k];
-      c += orient_keep(dv, v, voff[w + 1] - voff[w], w);
-    }
-    c = wave_sum32(c);
-    if (lane == 0) ocnt[v] = c;
-  }
-}
-__global__ __launch_bounds__(256) void k_orient_fill(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
-                                                     int64_t nv, const int64_t* __restrict__ ooff,
-                                                     int32_t* __restrict__ oind) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int64_t v = wave; v < nv; v += nwaves) {
-    const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-    int64_t pos = ooff[v];
-    for (int64_t k0 = b; k0 < e; k0 += 64) {
-      const int64_t k = k0 + lane;
-      bool keep = false;
-      int32_t w = 0;
-      if (k < e) {
-        w = ind[k];
-        keep = orient_keep(dv, v, voff[w + 1] - voff[w], w);
-      }
-      const uint64_t m = __ballot(keep);
-      if (keep) oind[pos + __popcll(m & lt)] = w;
-      pos += __popcll(m);
-    }
-  }
 }
 
 // ---------------------------------------------------------------- count
@@ -399,9 +361,11 @@ __device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
 // [g0, g1), that occur in the sorted list A[0, lA) held in LDS with
 // A[lA] = INT_MAX.  D chunks are in flight per lane: chunk c + D is loaded
 // into the register slot chunk c is compared from.
-template <int CH, int D>
+// With XC, a match at A position p also adds 2 to X[p] (LDS).
+template <int CH, int D, bool XC = false>
 __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
-                                                 int64_t g0, int64_t g1, const int32_t* A, int32_t lA) {
+                                                 int64_t g0, int64_t g1, const int32_t* A, int32_t lA,
+                                                 int32_t* X = nullptr) {
   constexpr int Q = CH / 4;
   if (c_lo >= c_hi) return 0;
   const int4* b4 = reinterpret_cast<const int4*>(base);
@@ -452,7 +416,14 @@ __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ bas
         for (int j = 0; j < CH; ++j) {
           const int32_t bv = b[j];
           while (acur < bv) acur = A[++pos];
-          cnt += (acur == bv);
+          if constexpr (XC) {
+            if (acur == bv) {
+              ++cnt;
+              atomicAdd(&X[pos], 2);
+            }
+          } else {
+            cnt += (acur == bv);
+          }
         }
       }
     }
@@ -501,9 +472,10 @@ __device__ __forceinline__ bool edge_search(int32_t lA, int32_t lB, int32_t rati
 // about CH compares per A value and no branch per B element, so it wins
 // when A is much shorter than B (few A values per chunk).  Needs
 // A[lA] = INT_MAX (a value above every chunk element).
-template <int CH, int D>
+template <int CH, int D, bool XC = false>
 __device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
-                                                   int64_t g0, int64_t g1, const int32_t* A, int32_t lA) {
+                                                   int64_t g0, int64_t g1, const int32_t* A, int32_t lA,
+                                                   int32_t* X = nullptr) {
   constexpr int Q = CH / 4;
   if (c_lo >= c_hi) return 0;
   const int4* b4 = reinterpret_cast<const int4*>(base);
@@ -561,6 +533,9 @@ __device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ b
 #pragma unroll
           for (int j = 0; j < CH; ++j) hit |= b[j] == acur;
           cnt += hit;
+          if constexpr (XC) {
+            if (hit) atomicAdd(&X[pos], 2);
+          }
           ++pos;
           acur = anext;
         }
@@ -571,6 +546,7 @@ __device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ b
 }
 
 #include "tc_bs_kernel.h"
+#include "tc_orient_kernel.h"
 #include "tc_search_kernel.h"
 
 // One lane per edge, two-pointer merge from global memory.  Simple and
@@ -819,6 +795,34 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
   return hipGetLastError();
 }
 
+constexpr int kOrientAW = 1019;
+int resident_orient_wgs(int cus) {
+  static std::mutex mu;
+  static int occ = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!occ) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k_count_orient<kOrientAW>), 256,
+                                                     0) != hipSuccess ||
+        n < 1)
+      n = 1;
+    occ = n;
+  }
+  return occ * cus;
+}
+
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio, int cus,
+                               hipStream_t s) {
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
+  hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
+  if (e != hipSuccess) return e;
+  if (nv <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_orient<kOrientAW>, dim3(resident_orient_wgs(cus)), dim3(256), 0, s,
+                     (const int64_t*)p.ooff, (const int32_t*)p.oind, nv, out,
+                     reinterpret_cast<unsigned long long*>(total), ctr, probe_ratio);
+  return hipGetLastError();
+}
+
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {
   if (nlist <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_mark_list, dim3(grid_for(nlist, 256, 16384)), dim3(256), 0, s, list, nlist, nv, vmark);
@@ -833,13 +837,21 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
 }
 
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
-  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 32);
-  if (nv > 0) hipLaunchKernelGGL(k_orient_count, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, nv, p.ecnt);
-  hipError_t e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.ooff, p.scan_tmp, s);
+  // the edge-list kernels in mode 3: counts per row into ecnt, the oriented
+  // row offsets ooff by a scan, then the kept ids into oind
+  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 16);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
+  hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
-    hipLaunchKernelGGL(k_orient_fill, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, nv,
-                       (const int64_t*)p.ooff, p.oind);
+    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr,
+                       nv, 3, (const int32_t*)nullptr, p.ecnt, ctr);
+  e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.ooff, p.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  if (nv > 0)
+    hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr, nv,
+                       3, (const int32_t*)nullptr, (const int64_t*)p.ooff, (int64_t)p.ocap, (int32_t*)nullptr, p.oind,
+                       (int32_t*)nullptr, ctr + 1);
   return hipGetLastError();
 }
 

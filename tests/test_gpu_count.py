@@ -51,6 +51,36 @@ def test_random_graphs(counter, g):
     _check(counter, g)
 
 
+_ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1, probe_ratio=-1)]
+
+
+@pytest.mark.parametrize("cfg", _ORIENTED, ids=str)
+def test_oriented(counter, cfg):
+    """Degree-oriented per-vertex counting (k_count_orient): each triangle
+    found once and its three vertices credited; same counts as the oracle."""
+    for g in GR.small_cases() + GR.random_cases() + _search_graphs():
+        _check(counter, g, cfg=cfg)
+
+
+def test_oriented_long_rows(counter):
+    """An oriented row longer than one wave's LDS window (K_1100: the lowest
+    vertex has 1099 out-neighbours) is walked in windows; closed form
+    out[v] = (n-1)(n-2)."""
+    n = 1100
+    g = GR.complete(n)
+    for cfg in _ORIENTED:
+        out, tot = counter.count_graph(g, cfg=cfg)
+        assert (out == (n - 1) * (n - 2)).all()
+        assert tot == n * (n - 1) * (n - 2)
+
+
+def test_oriented_only_for_algo_0(counter):
+    """Other algos count the full lists whatever cfg.oriented says."""
+    g = G.rmat(11, 16, seed=1)
+    for algo in (1, 2, 3):
+        _check(counter, g, cfg=dict(oriented=1, algo=algo))
+
+
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: "x".join(map(str, v)))
 @pytest.mark.parametrize("small_max", [16, 300, 2048, 0])
 @pytest.mark.parametrize("ranges", [1, 32])
