@@ -51,8 +51,9 @@ tc_cfg builtin_default() {
   c.depth = 2;
   c.algo = 0;
   c.search_ratio = 64;
-  c.probe_ratio = 4;
+  c.probe_ratio = 0;  // the scheme's own: 4 on the edge-list scheme, off on the oriented one
   c.oriented = 1;
+  c.row_groups = 4;
   return c;
 }
 
@@ -70,7 +71,8 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->depth) c.depth = in->depth;
     if (in->search_ratio) c.search_ratio = in->search_ratio;
     if (in->probe_ratio) c.probe_ratio = in->probe_ratio;
-    if (in->oriented) c.oriented = in->oriented > 0 ? 1 : 0;
+    if (in->oriented) c.oriented = in->oriented > 0 ? (in->oriented > 1 ? 2 : 1) : 0;
+    if (in->row_groups) c.row_groups = in->row_groups;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -82,6 +84,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want && stream_ok) vi = i;
   }
   if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 3) return -1;
+  if (c.row_groups != 1 && c.row_groups != 2 && c.row_groups != 4 && c.row_groups != 8) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   *out = c;
@@ -90,12 +93,13 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 
 // Scratch layout shared by the single-graph, batch and total-only paths.
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, vmark, vcnt, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, ioff, iind, prow, vmark, vcnt,
+      total;
   int64_t ecap, ocap;
-  bool list;
+  bool list, rev;
 };
 
-Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = false) {
+Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = false, bool rev = false) {
   Layout L{};
   size_t p = 0;
   auto take = [&](size_t bytes) {
@@ -123,6 +127,10 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = fals
   L.scan = take(8 * tcd::scan_tmp_elems(ncap));
   L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
   L.oind = take(orient ? 4 * (size_t)L.ocap : 0);
+  L.rev = orient && rev;
+  L.ioff = take(L.rev ? 8 * (size_t)(nv + 1) : 0);
+  L.iind = take(L.rev ? 4 * (size_t)L.ocap : 0);
+  L.prow = take(orient ? 4 * (size_t)(L.ocap / tcd::kOrientPiece + 2) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
@@ -167,6 +175,9 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
   p.oind = L.ocap ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
   p.ocap = L.ocap;
+  p.ioff = L.rev ? reinterpret_cast<int64_t*>(b + L.ioff) : nullptr;
+  p.iind = L.rev ? reinterpret_cast<int32_t*>(b + L.iind) : nullptr;
+  p.prow = L.ocap ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
   p.vmark = L.list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
   p.vcnt = L.list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
@@ -203,7 +214,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.qctr = p.qctr;
   a.search_ratio = (cfg.algo == 0 || cfg.algo == 3) && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
   a.u_only = u_only ? 1 : 0;
-  a.probe_ratio = cfg.probe_ratio > 0 ? cfg.probe_ratio : 0;
+  a.probe_ratio = cfg.probe_ratio > 0 ? cfg.probe_ratio : (cfg.probe_ratio == 0 ? 4 : 0);
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
   int nwg3 = tcd::resident_search_wgs(c->cus);
@@ -339,7 +350,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   if (cfg.algo != 0) cfg.oriented = 0;  // the oriented kernel is algo 0's
-  const Layout L = layout(nv, nnz, false, cfg.oriented != 0);
+  const Layout L = layout(nv, nnz, false, cfg.oriented != 0, false, cfg.oriented == 2);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, false);
@@ -354,9 +365,11 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    if (cfg.oriented == 2 && (e = tcd::launch_orient_in(p, nv, d_ind, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "orient");
     record(c, 1, s);
-    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.probe_ratio > 0 ? cfg.probe_ratio : 0, c->cus, s)) !=
-        hipSuccess)
+    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.probe_ratio > 0 ? cfg.probe_ratio : 0, cfg.row_groups,
+                                      cfg.oriented == 2, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)

@@ -39,6 +39,9 @@ struct Variant {
 int num_variants();
 Variant variant(int i);
 
+// Partner arcs per work piece of the oriented count kernel.
+constexpr int64_t kOrientPiece = 256;
+
 // Scratch sizing and prep/count launchers (stream-ordered, no host sync).
 struct Prep {
   int64_t* voff;    // [nv + 1]
@@ -59,6 +62,11 @@ struct Prep {
   int64_t* ooff;    // [nv + 1]
   int32_t* oind;    // [ocap]
   int64_t ocap;
+  // in-neighbour rows of the orientation (row-by-higher-vertex counting)
+  int64_t* ioff;    // [nv + 1] or nullptr
+  int32_t* iind;    // [ocap]
+  // row of the first partner arc of every piece of kOrientPiece arcs
+  int32_t* prow;    // [ocap / kOrientPiece + 2]
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -73,6 +81,8 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
                               int cus, hipStream_t s);
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
+// in-neighbour rows of the orientation into p.ioff / p.iind
+hipError_t launch_orient_in(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
@@ -84,8 +94,11 @@ int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
 // Per-vertex counts over the oriented CSR in p.ooff / p.oind (k_count_orient).
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio, int cus,
-                               hipStream_t s);
+// groups: arcs of a row streamed at a time per wave (1, 2, 4 or 8).  by_in:
+// rows are the higher vertex of each arc (partners from p.ioff / p.iind)
+// instead of the lower one (partners are the row's own out-neighbours).
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio,
+                               int groups, bool by_in, int cus, hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

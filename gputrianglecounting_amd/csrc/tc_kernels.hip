@@ -83,6 +83,13 @@ __global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
     p[i] = v;
 }
 
+// Row offsets past an array's capacity (a CSR that is not symmetric keeps
+// more arcs than the capacity holds) are cut to it, so readers stay inside.
+__global__ void k_clamp_i64(int64_t* p, int64_t n, int64_t cap) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (p[i] > cap) p[i] = cap;
+}
+
 __global__ void k_voff_from_off32(const int32_t* __restrict__ off, int64_t n1, int64_t* __restrict__ voff) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x)
     voff[i] = off[i];
@@ -198,14 +205,16 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
   return dv < dw || (dv == dw && v < w);
 }
 // mode 3 (orientation): keep arc v -> w iff (d(v), v) < (d(w), w); the
-// fill then writes only w, building the oriented CSR row by row.
+// fill then writes only w, building the oriented CSR row by row.  mode 4
+// keeps the other arcs of each row: v's in-neighbours in the orientation.
 __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
   if (mode == 0) return wl > vl && dv >= 2 && dw >= 2;
   if (mode == 2) return wl != vl && dv >= 2 && dw >= 2;
   if (mode == 3) return orient_keep(dv, vl, dw, wl);
+  if (mode == 4) return orient_keep(dw, wl, dv, vl);
   return dv >= 1 && dw >= 1;
 }
-__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 || mode == 3 ? 1 : 2; }
+__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 || mode >= 3 ? 1 : 2; }
 
 // Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
 // so hub rows do not pile up on one wave; each lane keeps 4 arc loads in
@@ -298,7 +307,7 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
             const int64_t idx = pos + __popcll(m & lt);
             if (idx < ecap) {
               ew[idx] = (int32_t)((int64_t)vb + wl[r]);
-              if (mode != 3) {
+              if (mode < 3) {
                 eu[idx] = (int32_t)v;
                 ework[idx] = (int32_t)(dv + dw);
               }
@@ -795,30 +804,56 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
   return hipGetLastError();
 }
 
-constexpr int kOrientAW = 1019;
-int resident_orient_wgs(int cus) {
-  static std::mutex mu;
-  static int occ = 0;
-  std::lock_guard<std::mutex> lk(mu);
-  if (!occ) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k_count_orient<kOrientAW>), 256,
-                                                     0) != hipSuccess ||
-        n < 1)
-      n = 1;
-    occ = n;
+// prow[m] = the row r with toff[r] <= m * P < toff[r + 1], for every piece
+// m of P partner arcs.
+__global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64_t P, int32_t* __restrict__ prow) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nv; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t lo = toff[r], hi = toff[r + 1];
+    for (int64_t m = (lo + P - 1) / P; m * P < hi; ++m) prow[m] = (int32_t)r;
   }
-  return occ * cus;
 }
 
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio, int cus,
-                               hipStream_t s) {
+constexpr int kOrientAW = 1019;
+using OrientFn = void (*)(const int64_t*, const int32_t*, const int64_t*, const int32_t*, const int32_t*, int64_t,
+                          int32_t*, unsigned long long*, unsigned long long*, int32_t);
+// arcs per wave at a time (groups) -> kernel
+OrientFn orient_fn(int groups) {
+  constexpr int P = (int)kOrientPiece;
+  switch (groups) {
+    case 1: return k_count_orient<kOrientAW, 1, P>;
+    case 2: return k_count_orient<kOrientAW, 2, P>;
+    case 8: return k_count_orient<kOrientAW, 8, P>;
+    default: return k_count_orient<kOrientAW, 4, P>;
+  }
+}
+int resident_orient_wgs(int groups, int cus) {
+  static std::mutex mu;
+  static int occ[9] = {};
+  const int gi = groups == 1 || groups == 2 || groups == 8 ? groups : 4;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!occ[gi]) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(orient_fn(gi)), 256, 0) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    occ[gi] = n;
+  }
+  return occ[gi] * cus;
+}
+
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio,
+                               int groups, bool by_in, int cus, hipStream_t s) {
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_count_orient<kOrientAW>, dim3(resident_orient_wgs(cus)), dim3(256), 0, s,
-                     (const int64_t*)p.ooff, (const int32_t*)p.oind, nv, out,
+  const int64_t* toff = by_in ? p.ioff : p.ooff;
+  const int32_t* tind = by_in ? p.iind : p.oind;
+  if (!toff || !tind || !p.prow) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, toff, nv, kOrientPiece, p.prow);
+  hipLaunchKernelGGL(orient_fn(groups), dim3(resident_orient_wgs(groups, cus)), dim3(256), 0, s,
+                     (const int64_t*)p.ooff, (const int32_t*)p.oind, toff, tind, (const int32_t*)p.prow, nv, out,
                      reinterpret_cast<unsigned long long*>(total), ctr, probe_ratio);
   return hipGetLastError();
 }
@@ -836,23 +871,32 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
   return hipGetLastError();
 }
 
-hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
-  // the edge-list kernels in mode 3: counts per row into ecnt, the oriented
-  // row offsets ooff by a scan, then the kept ids into oind
+// Rows of the orientation (mode 3, out-neighbours, into ooff/oind) or of
+// its reverse (mode 4, in-neighbours, into roff/rind).
+hipError_t launch_orient_rows(const Prep& p, int64_t nv, const int32_t* ind, int mode, int64_t* roff, int32_t* rind,
+                              int64_t cap, int cus, hipStream_t s) {
   const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 16);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr,
-                       nv, 3, (const int32_t*)nullptr, p.ecnt, ctr);
-  e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.ooff, p.scan_tmp, s);
+                       nv, mode, (const int32_t*)nullptr, p.ecnt, ctr);
+  e = scan_excl<int32_t>(p.ecnt, nullptr, nv, roff, p.scan_tmp, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr, nv,
-                       3, (const int32_t*)nullptr, (const int64_t*)p.ooff, (int64_t)p.ocap, (int32_t*)nullptr, p.oind,
+                       mode, (const int32_t*)nullptr, (const int64_t*)roff, cap, (int32_t*)nullptr, rind,
                        (int32_t*)nullptr, ctr + 1);
+  // fill wrote only positions below cap; rows are cut to match
+  hipLaunchKernelGGL(k_clamp_i64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, roff, nv + 1, cap);
   return hipGetLastError();
+}
+hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
+  return launch_orient_rows(p, nv, ind, 3, p.ooff, p.oind, p.ocap, cus, s);
+}
+hipError_t launch_orient_in(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
+  return launch_orient_rows(p, nv, ind, 4, p.ioff, p.iind, p.ocap, cus, s);
 }
 
 int occupancy(void (*fn)(CountArgs), int bs) {

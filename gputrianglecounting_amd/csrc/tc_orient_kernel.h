@@ -6,28 +6,45 @@
 // the arc a -> b.  Crediting 2 to each of a, b and c per triangle gives
 // out[v] = 2 T(v), the reference's per-vertex value, and the total 6 T.
 //
-// One wave takes a whole oriented row u at a time (rows are handed out in
-// small chunks from a counter): N+(u) is staged in the wave's LDS slice A
-// next to a counter array X, and for every w in N+(u) the wave streams
-// N+(w) against A exactly as the streamed count kernel does.  A match at A
-// position p adds 2 to X[p] (each A element matches at most one element of
-// one N+(w), so positions never collide within an edge); after the row,
-// one atomic per nonzero X[p] credits vertex A[p].  A vertex that closes
-// many triangles (a hub, which has the highest rank and so sits in many
-// rows) thus gets about one atomic per row, not one per triangle.  out[w]
-// gets 2 |N+(u) ∩ N+(w)| per arc and out[u] the row's sum.
+// Every row r has partners p from toff/tind: either its own out-neighbours
+// (toff/tind = ooff/oind, arcs r -> p) or its in-neighbours (toff/tind =
+// the reversed orientation, arcs p -> r).  Every arc is the pair (r, p) of
+// exactly one row either way, and the arc's matches are N+(r) ∩ N+(p).  The
+// second form streams N+ of the lower end of each arc, which under degree
+// order is the shorter list on average, but a hub row then has as many
+// partners as the hub has lower neighbours.  Work is therefore handed out
+// in pieces of P consecutive partner arcs (counter ctr); prow[b] is the row
+// holding arc b * P, and a piece walks the rows it overlaps, so a hub row
+// spreads over many waves and a run of short rows shares one.
 //
-// A row longer than AW is walked in windows of AW elements of N+(u); every
-// N+(w) is then cut to the window's value range with a wave-wide search,
+// N+(r) is staged in the wave's LDS slice A next to a counter array X, and
+// for every partner p the wave streams N+(p) against A exactly as the
+// streamed count kernel does.  A match at A position y adds 2 to X[y]
+// (each A element matches at most one element of one N+(p), so positions
+// never collide within an arc); after the row, one atomic per nonzero X[y]
+// credits vertex A[y].  A vertex that closes many triangles (a hub, which
+// has the highest rank and so sits in many rows) thus gets about one atomic
+// per row, not one per triangle.  out[p] gets 2 |N+(r) ∩ N+(p)| per arc and
+// out[r] the row's sum.
+//
+// A row longer than AW is walked in windows of AW elements of N+(r); every
+// N+(p) is then cut to the window's value range with a wave-wide search,
 // so each match still falls in exactly one window.
-template <int AW>
+//
+// Rows that fit one window take G partners at a time: the wave splits into
+// G groups of 64/G lanes, group g streaming N+(p) of partner j + g, so
+// short N+(p) (the usual case) keep every lane busy.  Groups may hit the
+// same A position; X is updated with LDS atomics.
+template <int AW, int G, int P>
 __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict__ ooff,
-                                                      const int32_t* __restrict__ oind, int64_t nv,
+                                                      const int32_t* __restrict__ oind,
+                                                      const int64_t* __restrict__ toff,
+                                                      const int32_t* __restrict__ tind,
+                                                      const int32_t* __restrict__ prow, int64_t nv,
                                                       int32_t* __restrict__ out, unsigned long long* total,
                                                       unsigned long long* ctr, int32_t probe_ratio) {
   constexpr int NW = 4;
   constexpr int CH = 16, SD = 2;
-  constexpr int64_t kRows = 4;  // rows per grab
   __shared__ int32_t abuf_all[NW * (AW + 1)];
   __shared__ int32_t xcnt_all[NW * AW];
   const int lane = threadIdx.x & 63;
@@ -37,15 +54,28 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
   const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(oind) >> 2) & (CH - 1));
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
+  const int64_t M = toff[nv];  // partner arcs
   for (;;) {
-    int64_t u0 = 0;
-    if (lane == 0) u0 = (int64_t)atomicAdd(ctr, (unsigned long long)kRows);
-    u0 = uniform64(__shfl(u0, 0, 64));
-    if (u0 >= nv) break;
-    const int64_t u1 = u0 + kRows < nv ? u0 + kRows : nv;
-    for (int64_t u = u0; u < u1; ++u) {
+    int64_t b = 0;
+    if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
+    b = uniform64(__shfl(b, 0, 64));
+    const int64_t a0 = b * P;
+    if (a0 >= M) break;
+    const int64_t a1 = a0 + P < M ? a0 + P : M;
+    // rows overlapping [a0, a1): from prow[b] until one ends at or past a1
+    // (toff[nv] = M >= a1 stops the walk at row nv - 1 at the latest)
+    int64_t u = prow[b];
+    int64_t t_next = toff[u];
+    for (;; ++u) {
+      const int64_t t_lo = t_next;
+      t_next = toff[u + 1];
+      const int64_t tu = t_lo > a0 ? t_lo : a0;
+      const int64_t lt = (t_next < a1 ? t_next : a1) - tu;  // partners of row u in this piece
       const int64_t ou = ooff[u], lu = ooff[u + 1] - ou;
-      if (lu == 0) continue;
+      if (lu == 0 || lt <= 0) {
+        if (t_next >= a1) break;
+        continue;
+      }
       int64_t cu = 0;
       for (int64_t ws = 0; ws < lu; ws += AW) {
         const int32_t n = (int32_t)(lu - ws < AW ? lu - ws : AW);
@@ -59,38 +89,69 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const bool whole = n == lu;
         const int32_t amin = oind[ou + ws], amax = oind[ou + ws + n - 1];
-        // row offsets of w_j as 32-bit words in lanes 0-3, one w ahead
-        auto off_load = [&](int64_t j) -> int32_t {
-          if (j >= lu || lane >= 4) return 0;
-          const int32_t w = oind[ou + j];
-          return reinterpret_cast<const int32_t*>(ooff + w + (lane >> 1))[lane & 1];
-        };
-        int32_t offs = off_load(0);
-        for (int64_t j = 0; j < lu; ++j) {
-          const int32_t w = oind[ou + j];
-          const int64_t ow = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 0) |
-                             ((int64_t)__builtin_amdgcn_readlane(offs, 1) << 32);
-          const int64_t we = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 2) |
-                             ((int64_t)__builtin_amdgcn_readlane(offs, 3) << 32);
-          offs = off_load(j + 1);
-          const int64_t lw = we - ow;
-          int64_t g0 = ow + sh, g1 = we + sh;  // positions in ind4
-          if (lw > 0 && !whole) {
-            g0 = wave_lower_bound(ind4, g0, g1, amin, lane);
-            g1 = wave_lower_bound(ind4, g0, g1, amax == INT_MAX ? amax : amax + 1, lane);
+        if (G > 1 && whole) {
+          constexpr int LG = 64 / G;
+          const int gid = lane / LG, gl = lane % LG;
+          int64_t cg = 0;  // matches of this group's arcs (summed in every lane of the group)
+          for (int64_t j = 0; j < lt; j += G) {
+            const int64_t jj = j + gid;
+            int32_t c = 0;
+            if (jj < lt) {
+              const int32_t w = tind[tu + jj];
+              const int64_t ow = ooff[w], lw = ooff[w + 1] - ow;
+              const int64_t g0 = ow + sh, g1 = ow + lw + sh;
+              if (lw > 0) {
+                const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
+                const int64_t c_lo = cb + (nc * gl) / LG, c_hi = cb + (nc * (gl + 1)) / LG;
+                const bool by_a = probe_ratio > 0 && lw >= (int64_t)probe_ratio * n;
+                c = by_a ? stream_chunks_a<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X)
+                         : stream_chunks<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X);
+              }
+#pragma unroll
+              for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+              if (gl == 0 && c) atomicAdd(&out[w], 2 * c);
+            }
+            cg += c;
           }
-          int32_t c = 0;
-          if (g0 < g1) {
-            const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
-            const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
-            const bool by_a = probe_ratio > 0 && lw >= (int64_t)probe_ratio * n;
-            c = by_a ? stream_chunks_a<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X)
-                     : stream_chunks<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X);
-          }
-          c = wave_sum32(c);
-          if (c) {
-            if (lane == 0) atomicAdd(&out[w], 2 * c);
-            cu += c;
+          // row sum: one lane per group holds its group's total
+          int64_t rs = gl == 0 ? cg : 0;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
+          cu += uniform64(rs);
+        } else {
+          // row offsets of partner j as 32-bit words in lanes 0-3, one ahead
+          auto off_load = [&](int64_t j) -> int32_t {
+            if (j >= lt || lane >= 4) return 0;
+            const int32_t w = tind[tu + j];
+            return reinterpret_cast<const int32_t*>(ooff + w + (lane >> 1))[lane & 1];
+          };
+          int32_t offs = off_load(0);
+          for (int64_t j = 0; j < lt; ++j) {
+            const int32_t w = tind[tu + j];
+            const int64_t ow = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 0) |
+                               ((int64_t)__builtin_amdgcn_readlane(offs, 1) << 32);
+            const int64_t we = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 2) |
+                               ((int64_t)__builtin_amdgcn_readlane(offs, 3) << 32);
+            offs = off_load(j + 1);
+            const int64_t lw = we - ow;
+            int64_t g0 = ow + sh, g1 = we + sh;  // positions in ind4
+            if (lw > 0 && !whole) {
+              g0 = wave_lower_bound(ind4, g0, g1, amin, lane);
+              g1 = wave_lower_bound(ind4, g0, g1, amax == INT_MAX ? amax : amax + 1, lane);
+            }
+            int32_t c = 0;
+            if (g0 < g1) {
+              const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
+              const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
+              const bool by_a = probe_ratio > 0 && lw >= (int64_t)probe_ratio * n;
+              c = by_a ? stream_chunks_a<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X)
+                       : stream_chunks<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X);
+            }
+            c = wave_sum32(c);
+            if (c) {
+              if (lane == 0) atomicAdd(&out[w], 2 * c);
+              cu += c;
+            }
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -108,6 +169,7 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
         if (lane == 0) atomicAdd(&out[u], (int32_t)(2 * cu));
         tri += cu;
       }
+      if (t_next >= a1) break;
     }
   }
   if (lane == 0 && total && tri) atomicAdd(total, (unsigned long long)(6 * tri));

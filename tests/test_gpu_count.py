@@ -51,7 +51,10 @@ def test_random_graphs(counter, g):
     _check(counter, g)
 
 
-_ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1, probe_ratio=-1)]
+_ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1, probe_ratio=-1),
+             dict(oriented=1, row_groups=1), dict(oriented=1, row_groups=2), dict(oriented=1, row_groups=8),
+             dict(oriented=1, row_groups=8, probe_ratio=1),
+             dict(oriented=2), dict(oriented=2, row_groups=1), dict(oriented=2, row_groups=8, probe_ratio=1)]
 
 
 @pytest.mark.parametrize("cfg", _ORIENTED, ids=str)
@@ -72,6 +75,26 @@ def test_oriented_long_rows(counter):
         out, tot = counter.count_graph(g, cfg=cfg)
         assert (out == (n - 1) * (n - 2)).all()
         assert tot == n * (n - 1) * (n - 2)
+
+
+def test_one_way_arcs_stay_in_bounds(counter):
+    """A CSR that is not symmetric can keep more oriented arcs than a
+    symmetric one of the same size (here m leaves -> hub, hub -> two sinks:
+    m of m + 2 arcs point up), more than the orientation's capacity.  The
+    rows are cut to the capacity; the count runs and finds no triangle."""
+    import torch
+
+    m = 1000
+    off = np.concatenate([np.arange(m + 1), [m + 2, m + 2, m + 2]]).astype(np.int32)
+    ind = np.concatenate([np.full(m, m), [m + 1, m + 2]]).astype(np.int32)
+    d_off = torch.from_numpy(off).cuda()
+    d_ind = torch.from_numpy(ind).cuda()
+    assert counter.validate(d_off, d_ind) == T.TC_EGRAPH
+    for cfg in (dict(oriented=1), dict(oriented=2), dict(oriented=-1), dict(algo=3)):
+        out, tot = counter.count(d_off, d_ind, cfg=cfg)
+        torch.cuda.synchronize()
+        assert int(tot.item()) == 0 and int(out.abs().sum().item()) == 0
+    assert int(counter.count_total(d_off, d_ind).item()) == 0
 
 
 def test_oriented_only_for_algo_0(counter):
