@@ -51,7 +51,7 @@ tc_cfg builtin_default() {
   c.depth = 2;
   c.algo = 0;
   c.search_ratio = 64;
-  c.probe_ratio = 0;  // the scheme's own: 4 on the edge-list scheme, off on the oriented one
+  c.probe_ratio = 4;
   c.oriented = 1;
   c.row_groups = 4;
   return c;
@@ -214,7 +214,7 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   a.qctr = p.qctr;
   a.search_ratio = (cfg.algo == 0 || cfg.algo == 3) && cfg.search_ratio > 0 ? cfg.search_ratio : 0;
   a.u_only = u_only ? 1 : 0;
-  a.probe_ratio = cfg.probe_ratio > 0 ? cfg.probe_ratio : (cfg.probe_ratio == 0 ? 4 : 0);
+  a.probe_ratio = cfg.probe_ratio > 0 ? cfg.probe_ratio : 0;
   const int nwg = count_wg(c, cfg, vi);
   a.max_ranges = nwg * cfg.ranges_per_wg;
   int nwg3 = tcd::resident_search_wgs(c->cus);
@@ -368,8 +368,8 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     if (cfg.oriented == 2 && (e = tcd::launch_orient_in(p, nv, d_ind, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "orient");
     record(c, 1, s);
-    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.probe_ratio > 0 ? cfg.probe_ratio : 0, cfg.row_groups,
-                                      cfg.oriented == 2, c->cus, s)) != hipSuccess)
+    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.row_groups, cfg.oriented == 2, c->cus, s)) !=
+        hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)

@@ -97,8 +97,8 @@ hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int3
 // groups: arcs of a row streamed at a time per wave (1, 2, 4 or 8).  by_in:
 // rows are the higher vertex of each arc (partners from p.ioff / p.iind)
 // instead of the lower one (partners are the row's own out-neighbours).
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio,
-                               int groups, bool by_in, int cus, hipStream_t s);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
+                               int cus, hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

@@ -370,7 +370,14 @@ __device__ __forceinline__ int next_range(unsigned int* q, int nr, int xcd) {
 // [g0, g1), that occur in the sorted list A[0, lA) held in LDS with
 // A[lA] = INT_MAX.  D chunks are in flight per lane: chunk c + D is loaded
 // into the register slot chunk c is compared from.
-// With XC, a match at A position p also adds 2 to X[p] (LDS).
+// With XC, a match at A position p also adds 2 to the 16-bit counter p of
+// X (LDS, two counters per word, p & 1 selecting the half).
+__device__ __forceinline__ void x_credit(int32_t* X, int32_t pos) {
+  atomicAdd(&X[pos >> 1], 2 << ((pos & 1) << 4));
+}
+__device__ __forceinline__ int32_t x_read(const int32_t* X, int32_t pos) {
+  return (X[pos >> 1] >> ((pos & 1) << 4)) & 0xffff;
+}
 template <int CH, int D, bool XC = false>
 __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ base, int64_t c_lo, int64_t c_hi,
                                                  int64_t g0, int64_t g1, const int32_t* A, int32_t lA,
@@ -428,7 +435,7 @@ __device__ __forceinline__ int32_t stream_chunks(const int32_t* __restrict__ bas
           if constexpr (XC) {
             if (acur == bv) {
               ++cnt;
-              atomicAdd(&X[pos], 2);
+              x_credit(X, pos);
             }
           } else {
             cnt += (acur == bv);
@@ -543,7 +550,7 @@ __device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ b
           for (int j = 0; j < CH; ++j) hit |= b[j] == acur;
           cnt += hit;
           if constexpr (XC) {
-            if (hit) atomicAdd(&X[pos], 2);
+            if (hit) x_credit(X, pos);
           }
           ++pos;
           acur = anext;
@@ -813,17 +820,22 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
   }
 }
 
-constexpr int kOrientAW = 1019;
+// Window of N+(r) per wave, next-chunk prefetch and waves per SIMD of the
+// oriented count kernel: the window sets the LDS per workgroup (24 (AW + 1)
+// bytes), which with the registers bounds the resident waves.
+constexpr int kOrientAW = 851;
+constexpr bool kOrientPF = false;
+constexpr int kOrientWPE = 8;
 using OrientFn = void (*)(const int64_t*, const int32_t*, const int64_t*, const int32_t*, const int32_t*, int64_t,
-                          int32_t*, unsigned long long*, unsigned long long*, int32_t);
+                          int32_t*, unsigned long long*, unsigned long long*);
 // arcs per wave at a time (groups) -> kernel
 OrientFn orient_fn(int groups) {
   constexpr int P = (int)kOrientPiece;
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, P>;
-    case 2: return k_count_orient<kOrientAW, 2, P>;
-    case 8: return k_count_orient<kOrientAW, 8, P>;
-    default: return k_count_orient<kOrientAW, 4, P>;
+    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE>;
+    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE>;
+    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE>;
+    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE>;
   }
 }
 int resident_orient_wgs(int groups, int cus) {
@@ -842,8 +854,8 @@ int resident_orient_wgs(int groups, int cus) {
   return occ[gi] * cus;
 }
 
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int32_t probe_ratio,
-                               int groups, bool by_in, int cus, hipStream_t s) {
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
+                               int cus, hipStream_t s) {
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
@@ -854,7 +866,7 @@ hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t*
   hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, toff, nv, kOrientPiece, p.prow);
   hipLaunchKernelGGL(orient_fn(groups), dim3(resident_orient_wgs(groups, cus)), dim3(256), 0, s,
                      (const int64_t*)p.ooff, (const int32_t*)p.oind, toff, tind, (const int32_t*)p.prow, nv, out,
-                     reinterpret_cast<unsigned long long*>(total), ctr, probe_ratio);
+                     reinterpret_cast<unsigned long long*>(total), ctr);
   return hipGetLastError();
 }
 

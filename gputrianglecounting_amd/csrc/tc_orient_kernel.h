@@ -17,15 +17,15 @@
 // holding arc b * P, and a piece walks the rows it overlaps, so a hub row
 // spreads over many waves and a run of short rows shares one.
 //
-// N+(r) is staged in the wave's LDS slice A next to a counter array X, and
-// for every partner p the wave streams N+(p) against A exactly as the
-// streamed count kernel does.  A match at A position y adds 2 to X[y]
-// (each A element matches at most one element of one N+(p), so positions
-// never collide within an arc); after the row, one atomic per nonzero X[y]
-// credits vertex A[y].  A vertex that closes many triangles (a hub, which
-// has the highest rank and so sits in many rows) thus gets about one atomic
-// per row, not one per triangle.  out[p] gets 2 |N+(r) ∩ N+(p)| per arc and
-// out[r] the row's sum.
+// N+(r) is staged in the wave's LDS slice A next to 16-bit counters X (two
+// to a word), and for every partner p the wave streams N+(p) against A
+// exactly as the streamed count kernel does.  A match at A position y adds
+// 2 to X[y]; each A element matches at most one element of one N+(p), so
+// X[y] <= 2 P within a piece.  After the row's part of the piece, one
+// atomic per nonzero X[y] credits vertex A[y].  A vertex that closes many
+// triangles (a hub, which has the highest rank and so sits in many rows)
+// thus gets about one atomic per row, not one per triangle.  out[p] gets
+// 2 |N+(r) ∩ N+(p)| per arc and out[r] the row's sum.
 //
 // A row longer than AW is walked in windows of AW elements of N+(r); every
 // N+(p) is then cut to the window's value range with a wave-wide search,
@@ -35,22 +35,89 @@
 // G groups of 64/G lanes, group g streaming N+(p) of partner j + g, so
 // short N+(p) (the usual case) keep every lane busy.  Groups may hit the
 // same A position; X is updated with LDS atomics.
-template <int AW, int G, int P>
-__global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict__ ooff,
-                                                      const int32_t* __restrict__ oind,
-                                                      const int64_t* __restrict__ toff,
-                                                      const int32_t* __restrict__ tind,
-                                                      const int32_t* __restrict__ prow, int64_t nv,
-                                                      int32_t* __restrict__ out, unsigned long long* total,
-                                                      unsigned long long* ctr, int32_t probe_ratio) {
+//
+// Occupancy decides this kernel's speed (it waits on memory and LDS, and
+// the time falls in proportion to the waves per SIMD), so the lane's merge
+// is kept small: 32-bit chunk and element offsets from a 64-byte aligned
+// base, one chunk in registers (PF: plus the next one in flight).
+
+// Lane part of one list of the orientation: chunks [c0, c1) of CH elements
+// from bp (64-byte aligned); elements before offset e0 or from e1 on are
+// masked out.  A: the staged window (n elements, A[n] = INT_MAX).  Every
+// match at A position p adds 2 to X's 16-bit counter p; returns the
+// matches.
+template <int CH, bool PF>
+__device__ __forceinline__ int32_t orient_stream(const int32_t* __restrict__ bp, int32_t c0, int32_t c1, int32_t e0,
+                                                 int32_t e1, const int32_t* A, int32_t n, int32_t* X) {
+  constexpr int Q = CH / 4;
+  if (c0 >= c1) return 0;
+  const int4* b4 = reinterpret_cast<const int4*>(bp);
+  int4 nx[Q];
+  if (PF) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) nx[q] = b4[c0 * Q + q];
+  }
+  const int32_t bf = bp[c0 * CH > e0 ? c0 * CH : e0];
+  int32_t lo = 0, hi = n;  // lower bound of bf in A
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (A[mid] < bf) lo = mid + 1;
+    else hi = mid;
+  }
+  int32_t pos = lo;
+  int32_t acur = A[pos];
+  int32_t cnt = 0;
+  for (int32_t c = c0; c < c1; ++c) {
+    int32_t b[CH];
+    if (!PF) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) nx[q] = b4[c * Q + q];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      b[4 * q] = nx[q].x;
+      b[4 * q + 1] = nx[q].y;
+      b[4 * q + 2] = nx[q].z;
+      b[4 * q + 3] = nx[q].w;
+    }
+    if (PF && c + 1 < c1) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) nx[q] = b4[(c + 1) * Q + q];
+    }
+    const int32_t cb = c * CH;
+    if (cb < e0 || cb + CH > e1) {
+#pragma unroll
+      for (int j = 0; j < CH; ++j)
+        if (cb + j < e0 || cb + j >= e1) b[j] = -1;  // ids are >= 0: never matches, never advances
+    }
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int32_t bv = b[j];
+      while (acur < bv) acur = A[++pos];
+      if (acur == bv) {
+        ++cnt;
+        x_credit(X, pos);
+      }
+    }
+  }
+  return cnt;
+}
+
+template <int AW, int G, int P, bool PF, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
+    const int64_t* __restrict__ ooff, const int32_t* __restrict__ oind, const int64_t* __restrict__ toff,
+    const int32_t* __restrict__ tind, const int32_t* __restrict__ prow, int64_t nv, int32_t* __restrict__ out,
+    unsigned long long* total, unsigned long long* ctr) {
   constexpr int NW = 4;
-  constexpr int CH = 16, SD = 2;
+  constexpr int CH = 16;
+  constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
+  static_assert(2 * P < 65536, "a piece's counts must fit 16 bits");
   __shared__ int32_t abuf_all[NW * (AW + 1)];
-  __shared__ int32_t xcnt_all[NW * AW];
+  __shared__ int32_t xcnt_all[NW * XW];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int32_t* A = abuf_all + wv * (AW + 1);
-  int32_t* X = xcnt_all + wv * AW;
+  int32_t* X = xcnt_all + wv * XW;
   const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(oind) >> 2) & (CH - 1));
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
@@ -79,10 +146,8 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
       int64_t cu = 0;
       for (int64_t ws = 0; ws < lu; ws += AW) {
         const int32_t n = (int32_t)(lu - ws < AW ? lu - ws : AW);
-        for (int32_t y = lane; y < n; y += 64) {
-          A[y] = oind[ou + ws + y];
-          X[y] = 0;
-        }
+        for (int32_t y = lane; y < n; y += 64) A[y] = oind[ou + ws + y];
+        for (int32_t y = lane; y < (n + 1) / 2; y += 64) X[y] = 0;
         if (lane == 0) A[n] = INT_MAX;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -98,14 +163,12 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
             int32_t c = 0;
             if (jj < lt) {
               const int32_t w = tind[tu + jj];
-              const int64_t ow = ooff[w], lw = ooff[w + 1] - ow;
-              const int64_t g0 = ow + sh, g1 = ow + lw + sh;
-              if (lw > 0) {
-                const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
-                const int64_t c_lo = cb + (nc * gl) / LG, c_hi = cb + (nc * (gl + 1)) / LG;
-                const bool by_a = probe_ratio > 0 && lw >= (int64_t)probe_ratio * n;
-                c = by_a ? stream_chunks_a<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X)
-                         : stream_chunks<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X);
+              const int64_t ow = ooff[w], we = ooff[w + 1];
+              if (we > ow) {
+                const int64_t cb = (ow + sh) / CH;  // first chunk of N+(w) in ind4
+                const int32_t e0 = (int32_t)(ow + sh - cb * CH), e1 = (int32_t)(we + sh - cb * CH);
+                const int32_t nc = (e1 + CH - 1) / CH;
+                c = orient_stream<CH, PF>(ind4 + cb * CH, (nc * gl) / LG, (nc * (gl + 1)) / LG, e0, e1, A, n, X);
               }
 #pragma unroll
               for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -133,19 +196,17 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
             const int64_t we = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 2) |
                                ((int64_t)__builtin_amdgcn_readlane(offs, 3) << 32);
             offs = off_load(j + 1);
-            const int64_t lw = we - ow;
             int64_t g0 = ow + sh, g1 = we + sh;  // positions in ind4
-            if (lw > 0 && !whole) {
+            if (we > ow && !whole) {
               g0 = wave_lower_bound(ind4, g0, g1, amin, lane);
               g1 = wave_lower_bound(ind4, g0, g1, amax == INT_MAX ? amax : amax + 1, lane);
             }
             int32_t c = 0;
             if (g0 < g1) {
-              const int64_t cb = g0 / CH, nc = (g1 + CH - 1) / CH - cb;
-              const int64_t c_lo = cb + (nc * lane) / 64, c_hi = cb + (nc * (lane + 1)) / 64;
-              const bool by_a = probe_ratio > 0 && lw >= (int64_t)probe_ratio * n;
-              c = by_a ? stream_chunks_a<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X)
-                       : stream_chunks<CH, SD, true>(ind4, c_lo, c_hi, g0, g1, A, n, X);
+              const int64_t cb = g0 / CH;
+              const int32_t e0 = (int32_t)(g0 - cb * CH), e1 = (int32_t)(g1 - cb * CH);
+              const int32_t nc = (e1 + CH - 1) / CH;
+              c = orient_stream<CH, PF>(ind4 + cb * CH, (nc * lane) / 64, (nc * (lane + 1)) / 64, e0, e1, A, n, X);
             }
             c = wave_sum32(c);
             if (c) {
@@ -158,7 +219,7 @@ __global__ __launch_bounds__(256) void k_count_orient(const int64_t* __restrict_
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int32_t y = lane; y < n; y += 64) {
-          const int32_t x = X[y];
+          const int32_t x = x_read(X, y);
           if (x) atomicAdd(&out[A[y]], x);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

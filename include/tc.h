@@ -54,16 +54,15 @@ typedef struct tc_ctx tc_ctx;
  *                  2 = Merge Path through LDS rings for every long edge
  *   chunk          algo 0/3: elements per streamed chunk (16)
  *   depth          algo 0/3: streamed chunks in flight per lane (2 or 4)
- *   search_ratio   algo 0/3: an edge whose longer list B has at least 512 and
- *                  at least search_ratio times as many elements as the
- *                  shorter list A is counted by binary-searching every
- *                  element of A in B instead of streaming B (default 64;
- *                  negative: never)
- *   probe_ratio    algo 0/3: a streamed edge whose B is at least probe_ratio
- *                  times longer than A is merged from A's side, each A value
- *                  tested against a 16-element chunk of B in registers
- *                  (default: 4 on the edge-list scheme, never on the
- *                  oriented one; negative: never)
+ *   search_ratio   algo 0/3, edge-list scheme: an edge whose longer list B
+ *                  has at least 512 and at least search_ratio times as many
+ *                  elements as the shorter list A is counted by
+ *                  binary-searching every element of A in B instead of
+ *                  streaming B (default 64; negative: never)
+ *   probe_ratio    algo 0/3, edge-list scheme: a streamed edge whose B is at
+ *                  least probe_ratio times longer than A is merged from A's
+ *                  side, each A value tested against a 16-element chunk of B
+ *                  in registers (default 4; negative: never)
  *   oriented       tc_count_i32 with algo 0: 1 (default) = intersect the
  *                  degree-ordered orientation, one wave per oriented row,
  *                  each triangle found once and its three vertices credited;

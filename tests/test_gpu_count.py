@@ -66,10 +66,10 @@ def test_oriented(counter, cfg):
 
 
 def test_oriented_long_rows(counter):
-    """An oriented row longer than one wave's LDS window (K_1100: the lowest
-    vertex has 1099 out-neighbours) is walked in windows; closed form
-    out[v] = (n-1)(n-2)."""
-    n = 1100
+    """Oriented rows longer than one wave's LDS window (851 elements) are
+    walked in windows: in K_1800 the lowest vertices have up to 1799
+    out-neighbours (three windows); closed form out[v] = (n-1)(n-2)."""
+    n = 1800
     g = GR.complete(n)
     for cfg in _ORIENTED:
         out, tot = counter.count_graph(g, cfg=cfg)
