@@ -397,7 +397,14 @@ def main() -> int:
     inv_ok = bool((h_out % 2 == 0).all() and (h_out <= d * (d - 1)).all() and (h_out >= 0).all()
                   and tot == int(h_out.sum()) and tot % 6 == 0)
     tot_global = int(tc.count_total(off, ind).cpu().item())
-    log(f"checks: invariants={inv_ok} total={tot} global-mode total={tot_global}")
+    global_ms = tc.last_timing().count_ms
+    # an independent kernel: every undirected edge's full lists (no orientation)
+    e_out, e_tot = tc.count(off, ind, cfg=dict(cfg or {}, oriented=-1))
+    tot_edges = int(e_tot.cpu().item())
+    edges_identical = bool(torch.equal(e_out, out))
+    del e_out
+    log(f"checks: invariants={inv_ok} total={tot} global-mode total={tot_global} ({global_ms:.1f} ms) "
+        f"edge-scheme total={tot_edges} per-vertex identical={edges_identical}")
 
     torch.cuda.synchronize()
     kern = []
@@ -460,8 +467,10 @@ def main() -> int:
         "oriented": {k: ab[k] for k in ("oriented_arcs", "oriented_stream_elements", "max_out_degree")} if oriented
         else None,
         "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
+                   "global_mode_count_ms": round(global_ms, 3), "edge_scheme_total": tot_edges,
+                   "edge_scheme_per_vertex_identical": edges_identical,
                    "host_edge_count_agrees": True if oriented else ab["edges"] == E,
-                   "totals_agree": tot == tot_global, "triangles": tot // 6},
+                   "totals_agree": tot == tot_global == tot_edges, "triangles": tot // 6},
     }
     print(json.dumps(line), flush=True)
     tc.close()

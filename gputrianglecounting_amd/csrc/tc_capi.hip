@@ -87,19 +87,26 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
   if (c.row_groups != 1 && c.row_groups != 2 && c.row_groups != 4 && c.row_groups != 8) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
+  if (c.algo != 0) c.oriented = 0;  // the oriented kernel is algo 0's
   *out = c;
   return vi;
 }
 
-// Scratch layout shared by the single-graph, batch and total-only paths.
+// Scratch layout of one call.  The edge-list scheme keeps an undirected
+// edge list (at most nnz/2 edges for a symmetric CSR; in the vertex-list
+// mode every arc of a listed vertex is one, at most nnz); the oriented
+// scheme keeps the orientation in place of the input rows.
+enum class Scheme { kEdges, kList, kOrient };
+
+Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
+
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, ooff, oind, ioff, iind, prow, vmark, vcnt,
-      total;
-  int64_t ecap, ocap;
-  bool list, rev;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, vmark, vcnt, total;
+  int64_t ecap;
+  Scheme sc;
 };
 
-Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = false, bool rev = false) {
+Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   Layout L{};
   size_t p = 0;
   auto take = [&](size_t bytes) {
@@ -107,34 +114,32 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, bool orient, bool list = fals
     p += align_up(bytes);
     return at;
   };
-  // Undirected edges: at most nnz/2 for a symmetric CSR.  In the oriented
-  // mode every kept arc is an edge, again at most nnz/2; in the vertex-list
-  // mode every arc of a listed vertex is one, at most nnz.
-  L.list = list;
-  L.ecap = list ? nnz + 1 : nnz / 2 + 1;
-  L.ocap = orient ? nnz / 2 + 1 : 0;
+  const bool edges = sc != Scheme::kOrient, orient = sc == Scheme::kOrient, list = sc == Scheme::kList;
+  L.sc = sc;
+  L.ecap = edges ? (list ? nnz + 1 : nnz / 2 + 1) : 0;
   L.voff = take(8 * (size_t)(nv + 1));
   L.vbase = take(batch ? 4 * (size_t)(nv + 1) : 0);
-  L.ecnt = take(4 * (size_t)(nv + 1));
-  L.eoff = take(8 * (size_t)(nv + 1));
+  L.dscal = take(8 * 8);
+  L.qctr = take(4 * 24);
+  L.ecnt = take(edges ? 4 * (size_t)(nv + 1) : 0);
+  L.eoff = take(edges ? 8 * (size_t)(nv + 1) : 0);
   L.eu = take(4 * (size_t)L.ecap);
   L.ew = take(4 * (size_t)L.ecap);
   L.ework = take(4 * (size_t)L.ecap);
-  L.wpre = take(8 * (size_t)(L.ecap + 1));
-  L.dscal = take(8 * 8);
-  L.qctr = take(4 * 24);
-  const int64_t ncap = std::max<int64_t>(nv + 1, L.ecap);
-  L.scan = take(8 * tcd::scan_tmp_elems(ncap));
-  L.ooff = take(orient ? 8 * (size_t)(nv + 1) : 0);
-  L.oind = take(orient ? 4 * (size_t)L.ocap : 0);
-  L.rev = orient && rev;
-  L.ioff = take(L.rev ? 8 * (size_t)(nv + 1) : 0);
-  L.iind = take(L.rev ? 4 * (size_t)L.ocap : 0);
-  L.prow = take(orient ? 4 * (size_t)(L.ocap / tcd::kOrientPiece + 2) : 0);
+  L.wpre = take(edges ? 8 * (size_t)(L.ecap + 1) : 0);
+  L.scan = take(edges ? 8 * tcd::scan_tmp_elems(std::max<int64_t>(nv + 1, L.ecap)) : 0);
+  L.orow = take(orient ? 16 * (size_t)(nv + 1) : 0);
+  L.oind = take(orient ? 4 * (size_t)(nnz + 1) : 0);
+  L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
   return L;
+}
+
+// Scratch for a call whatever the scheme (for callers reserving up front).
+size_t layout_bound(int64_t nv, int64_t nnz, bool batch) {
+  return std::max(layout(nv, nnz, batch, Scheme::kEdges).total, layout(nv, nnz, batch, Scheme::kOrient).total);
 }
 
 int ensure(tc_ctx* c, size_t bytes) {
@@ -172,14 +177,12 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.qctr = reinterpret_cast<unsigned int*>(b + L.qctr);
   p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
   p.ecap = L.ecap;
-  p.ooff = L.ocap ? reinterpret_cast<int64_t*>(b + L.ooff) : nullptr;
-  p.oind = L.ocap ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
-  p.ocap = L.ocap;
-  p.ioff = L.rev ? reinterpret_cast<int64_t*>(b + L.ioff) : nullptr;
-  p.iind = L.rev ? reinterpret_cast<int32_t*>(b + L.iind) : nullptr;
-  p.prow = L.ocap ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
-  p.vmark = L.list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
-  p.vcnt = L.list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
+  const bool orient = L.sc == Scheme::kOrient, list = L.sc == Scheme::kList;
+  p.orow = orient ? reinterpret_cast<int64_t*>(b + L.orow) : nullptr;
+  p.oind = orient ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
+  p.prow = orient ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
+  p.vmark = list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
+  p.vcnt = list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
 }
 
@@ -269,9 +272,9 @@ int tc_ctx_destroy(tc_ctx* c) {
 
 const char* tc_last_error(const tc_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-size_t tc_scratch_bytes(int32_t nv, int64_t nnz) { return layout(nv, nnz, false, false).total; }
+size_t tc_scratch_bytes(int32_t nv, int64_t nnz) { return layout_bound(nv, nnz, false); }
 size_t tc_batch_scratch_bytes(int32_t G, int64_t nv_total, int64_t nnz_total) {
-  return layout(nv_total, nnz_total, true, false).total;
+  return layout_bound(nv_total, nnz_total, true);
 }
 
 int tc_ctx_reserve(tc_ctx* c, size_t bytes) {
@@ -349,8 +352,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  if (cfg.algo != 0) cfg.oriented = 0;  // the oriented kernel is algo 0's
-  const Layout L = layout(nv, nnz, false, cfg.oriented != 0, false, cfg.oriented == 2);
+  const Layout L = layout(nv, nnz, false, scheme_of(cfg));
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, false);
@@ -365,8 +367,6 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-    if (cfg.oriented == 2 && (e = tcd::launch_orient_in(p, nv, d_ind, c->cus, s)) != hipSuccess)
-      return hip_fail(c, e, "orient");
     record(c, 1, s);
     if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.row_groups, cfg.oriented == 2, c->cus, s)) !=
         hipSuccess)
@@ -391,27 +391,25 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || !d_total) return set_err(c, TC_EINVAL, "invalid argument");
   if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
   tc_cfg cfg;
-  const int vi = resolve(c, nullptr, &cfg);
-  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  if (resolve(c, nullptr, &cfg) < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv, nnz, false, true);
+  const Layout L = layout(nv, nnz, false, Scheme::kOrient);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, false);
   record(c, 0, s);
+  // The degree-ordered orientation finds every triangle once; without
+  // per-vertex output the count kernel keeps only the total (6 per
+  // triangle).  No edge list: tc_last_edge_work reports (0, 0).
   if ((e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-  // The oriented CSR becomes the graph: voff <- ooff, ind <- oind.
-  tcd::Prep q = p;
-  q.voff = p.ooff;
-  if ((e = tcd::launch_build_edges(q, nv, p.ooff, p.oind, 1, c->cus, s)) != hipSuccess)
-    return hip_fail(c, e, "edge list");
   record(c, 1, s);
-  rc = run_count(c, q, cfg, vi, p.oind, nullptr, d_total, 6, s);
-  if (rc) return rc;
+  if ((e = tcd::launch_count_orient(p, nv, nullptr, d_total, cfg.row_groups, false, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
   record(c, 3, s);
@@ -432,7 +430,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv, nnz, false, false, true);
+  const Layout L = layout(nv, nnz, false, Scheme::kList);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, false);
@@ -470,7 +468,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv_total, nnz_total, true, false);
+  const Layout L = layout(nv_total, nnz_total, true, Scheme::kEdges);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, true);
@@ -614,14 +612,18 @@ int capture(tc_ctx* c, F&& fn, tc_graph** out) {
 
 // Warm what a captured call must not do inside the capture: scratch
 // allocation and the occupancy query of the chosen kernel variant.
-int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, size_t bytes) {
+// Everything a capture must not do inside (allocation, occupancy queries)
+// done up front for a call of this size.
+int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bool batch) {
   tc_cfg cfg;
   const int vi = resolve(c, cfg_in, &cfg);
   if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
-  return ensure(c, bytes);
+  const Scheme sc = batch ? Scheme::kEdges : scheme_of(cfg);
+  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(cfg.row_groups, true, c->cus);
+  return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 
 struct Candidate {
@@ -744,7 +746,7 @@ int tc_batch_count_pergraph_i32(tc_ctx* c, int32_t G, const int64_t* h_tab, cons
     max_n = std::max(max_n, h_tab[2 * g + 2] - h_tab[2 * g]);
     max_nnz = std::max(max_nnz, h_tab[2 * g + 3] - h_tab[2 * g + 1]);
   }
-  int rc = ensure(c, layout(max_n, max_nnz, false, false).total);
+  int rc = ensure(c, layout_bound(max_n, max_nnz, false));
   if (rc) return rc;
   for (int32_t g = 0; g < G; ++g) {
     const int64_t vb = h_tab[2 * g], ab = h_tab[2 * g + 1];
@@ -760,7 +762,7 @@ int tc_count_capture_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_of
                          int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, tc_graph** out) {
   if (!c || !out) return TC_EINVAL;
   *out = nullptr;
-  int rc = prepare_capture(c, cfg, layout(nv, nnz, false, false).total);
+  int rc = prepare_capture(c, cfg, nv, nnz, false);
   if (rc) return rc;
   return capture(
       c, [&](hipStream_t s) { return tc_count_i32(c, nv, nnz, d_off, d_ind, d_out, d_total, cfg, s); }, out);
@@ -771,7 +773,7 @@ int tc_batch_capture_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_
                          const tc_cfg* cfg, tc_graph** out) {
   if (!c || !out) return TC_EINVAL;
   *out = nullptr;
-  int rc = prepare_capture(c, cfg, layout(nv_total, nnz_total, true, false).total);
+  int rc = prepare_capture(c, cfg, nv_total, nnz_total, true);
   if (rc) return rc;
   return capture(
       c,

@@ -58,15 +58,13 @@ struct Prep {
   unsigned int* qctr;  // [24]: three launches' per-XCD range queues
   int64_t* scan_tmp;
   int64_t ecap;
-  // oriented copy for the total-only mode
-  int64_t* ooff;    // [nv + 1]
-  int32_t* oind;    // [ocap]
-  int64_t ocap;
-  // in-neighbour rows of the orientation (row-by-higher-vertex counting)
-  int64_t* ioff;    // [nv + 1] or nullptr
-  int32_t* iind;    // [ocap]
-  // row of the first partner arc of every piece of kOrientPiece arcs
-  int32_t* prow;    // [ocap / kOrientPiece + 2]
+  // Degree-ordered orientation, in place of the input rows: row v keeps its
+  // arc positions voff[v] .. voff[v+1] in oind, N+(v) first (ascending),
+  // then the in-neighbours; orow[2v] = voff[v], orow[2v+1] = end of N+(v).
+  int64_t* orow;    // [2 nv] or nullptr
+  int32_t* oind;    // [nnz]
+  // row of the first arc of every piece of kOrientPiece arc positions
+  int32_t* prow;    // [nnz / kOrientPiece + 2]
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -80,23 +78,26 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 // mode 1: all arcs of an oriented CSR whose endpoints both have out-arcs
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
                               int cus, hipStream_t s);
+// Orientation in place (k_orient_rows) into p.orow / p.oind; ind holds
+// graph-local ids when p.vbase is set.
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
-// in-neighbour rows of the orientation into p.ioff / p.iind
-hipError_t launch_orient_in(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
 int resident_wgs2(int variant_idx, int cus, int which);
+// Resident workgroups of the oriented count kernel (per-vertex or total only).
+int resident_orient_wgs(int groups, bool pv, int cus);
 // nwg, nwg2, nwg3: grids of the streamed kernel's wave launch, its
 // workgroup launch and the lookup kernel (persistent: resident workgroups).
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
 int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
-// Per-vertex counts over the oriented CSR in p.ooff / p.oind (k_count_orient).
-// groups: arcs of a row streamed at a time per wave (1, 2, 4 or 8).  by_in:
-// rows are the higher vertex of each arc (partners from p.ioff / p.iind)
-// instead of the lower one (partners are the row's own out-neighbours).
+// Per-vertex counts (out; nullptr: the total only) over the orientation in
+// p.orow / p.oind (k_count_orient).  groups: arcs of a row streamed at a
+// time per wave (1, 2, 4 or 8).  by_in: rows are the higher vertex of each
+// arc (partners are its in-neighbours) instead of the lower one (partners
+// are its out-neighbours).
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
                                int cus, hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,

@@ -83,13 +83,6 @@ __global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
     p[i] = v;
 }
 
-// Row offsets past an array's capacity (a CSR that is not symmetric keeps
-// more arcs than the capacity holds) are cut to it, so readers stay inside.
-__global__ void k_clamp_i64(int64_t* p, int64_t n, int64_t cap) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    if (p[i] > cap) p[i] = cap;
-}
-
 __global__ void k_voff_from_off32(const int32_t* __restrict__ off, int64_t n1, int64_t* __restrict__ voff) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x)
     voff[i] = off[i];
@@ -201,20 +194,12 @@ hipError_t scan_excl(const Tin* in, const int64_t* n_dev, int64_t n_cap, int64_t
 // with d+(v) >= 1 and d+(w) >= 1.  mode 2 (vertex list): every arc v -> w of
 // a marked v with w != v, d(v) >= 2 and d(w) >= 2 (the reference's per-arc
 // rule, clusteringCount.cu:295-298).
-__device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, int64_t w) {
-  return dv < dw || (dv == dw && v < w);
-}
-// mode 3 (orientation): keep arc v -> w iff (d(v), v) < (d(w), w); the
-// fill then writes only w, building the oriented CSR row by row.  mode 4
-// keeps the other arcs of each row: v's in-neighbours in the orientation.
 __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
   if (mode == 0) return wl > vl && dv >= 2 && dw >= 2;
   if (mode == 2) return wl != vl && dv >= 2 && dw >= 2;
-  if (mode == 3) return orient_keep(dv, vl, dw, wl);
-  if (mode == 4) return orient_keep(dw, wl, dv, vl);
   return dv >= 1 && dw >= 1;
 }
-__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 || mode >= 3 ? 1 : 2; }
+__device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 ? 1 : 2; }
 
 // Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
 // so hub rows do not pile up on one wave; each lane keeps 4 arc loads in
@@ -306,15 +291,76 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
           if (keep) {
             const int64_t idx = pos + __popcll(m & lt);
             if (idx < ecap) {
+              eu[idx] = (int32_t)v;
               ew[idx] = (int32_t)((int64_t)vb + wl[r]);
-              if (mode < 3) {
-                eu[idx] = (int32_t)v;
-                ework[idx] = (int32_t)(dv + dw);
-              }
+              ework[idx] = (int32_t)(dv + dw);
             }
           }
           pos += __popcll(m);
         }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ orientation
+// Arc v -> w is kept iff (d(v), v) < (d(w), w) (ids graph-local in a batch,
+// where the order only ever compares vertices of one graph).
+__device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, int64_t w) {
+  return dv < dw || (dv == dw && v < w);
+}
+
+// Orientation in place, one wave per vertex (chunks from ctr): the arcs of
+// row v are written back to oind at the row's own positions voff[v] ..
+// voff[v+1], the kept ones N+(v) from the front in ascending order, the
+// others (v's in-neighbours) from the back; orow[2v] = voff[v] and
+// orow[2v+1] = voff[v] + |N+(v)|.  One pass, no scan, and a row never
+// spills into another whatever the input (a CSR that is not symmetric
+// only orients differently).
+__global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                     const int32_t* __restrict__ vbase, int64_t nv,
+                                                     int32_t* __restrict__ oind, int64_t* __restrict__ orow,
+                                                     unsigned long long* ctr) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (;;) {
+    const int64_t v0 = grab_vertices(ctr);
+    if (v0 >= nv) break;
+    const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
+    for (int64_t v = v0; v < v1; ++v) {
+      const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
+      const int32_t vb = vbase ? vbase[v] : 0;
+      const int32_t vl = (int32_t)(v - vb);
+      int64_t pf = b, pb = e;  // next front position, one past the next back one
+      for (int64_t k0 = b; k0 < e; k0 += 256) {
+        int32_t wl[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t k = k0 + lane + 64 * r;
+          wl[r] = k < e ? ind[k] : -1;
+        }
+        int64_t lo[4], hi[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
+          lo[r] = voff[wg];
+          hi[r] = voff[wg + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool valid = wl[r] >= 0;
+          const bool keep = valid && orient_keep(dv, vl, hi[r] - lo[r], wl[r]);
+          const uint64_t mk = __ballot(keep), mi = __ballot(valid && !keep);
+          const int32_t wg = (int32_t)((int64_t)vb + wl[r]);
+          if (keep) oind[pf + __popcll(mk & lt)] = wg;
+          else if (valid) oind[pb - 1 - __popcll(mi & lt)] = wg;
+          pf += __popcll(mk);
+          pb -= __popcll(mi);
+        }
+      }
+      if (lane == 0) {
+        orow[2 * v] = b;
+        orow[2 * v + 1] = pf;
       }
     }
   }
@@ -826,32 +872,34 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
 constexpr int kOrientAW = 851;
 constexpr bool kOrientPF = false;
 constexpr int kOrientWPE = 8;
-using OrientFn = void (*)(const int64_t*, const int32_t*, const int64_t*, const int32_t*, const int32_t*, int64_t,
-                          int32_t*, unsigned long long*, unsigned long long*);
-// arcs per wave at a time (groups) -> kernel
-OrientFn orient_fn(int groups) {
+using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, int32_t*,
+                          unsigned long long*, unsigned long long*, int);
+// arcs per wave at a time (groups), per-vertex or total only -> kernel
+template <bool PV>
+OrientFn orient_fn_pv(int groups) {
   constexpr int P = (int)kOrientPiece;
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE>;
-    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE>;
-    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE>;
-    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE>;
+    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE, PV>;
+    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE, PV>;
+    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE, PV>;
+    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE, PV>;
   }
 }
-int resident_orient_wgs(int groups, int cus) {
+OrientFn orient_fn(int groups, bool pv) { return pv ? orient_fn_pv<true>(groups) : orient_fn_pv<false>(groups); }
+int resident_orient_wgs(int groups, bool pv, int cus) {
   static std::mutex mu;
-  static int occ[9] = {};
+  static int occ[2][9] = {};
   const int gi = groups == 1 || groups == 2 || groups == 8 ? groups : 4;
   std::lock_guard<std::mutex> lk(mu);
-  if (!occ[gi]) {
+  if (!occ[pv][gi]) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(orient_fn(gi)), 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(orient_fn(gi, pv)), 256, 0) !=
             hipSuccess ||
         n < 1)
       n = 1;
-    occ[gi] = n;
+    occ[pv][gi] = n;
   }
-  return occ[gi] * cus;
+  return occ[pv][gi] * cus;
 }
 
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
@@ -860,13 +908,13 @@ hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t*
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
-  const int64_t* toff = by_in ? p.ioff : p.ooff;
-  const int32_t* tind = by_in ? p.iind : p.oind;
-  if (!toff || !tind || !p.prow) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, toff, nv, kOrientPiece, p.prow);
-  hipLaunchKernelGGL(orient_fn(groups), dim3(resident_orient_wgs(groups, cus)), dim3(256), 0, s,
-                     (const int64_t*)p.ooff, (const int32_t*)p.oind, toff, tind, (const int32_t*)p.prow, nv, out,
-                     reinterpret_cast<unsigned long long*>(total), ctr);
+  if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
+  const bool pv = out != nullptr;
+  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
+                     kOrientPiece, p.prow);
+  hipLaunchKernelGGL(orient_fn(groups, pv), dim3(resident_orient_wgs(groups, pv, cus)), dim3(256), 0, s,
+                     (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.prow, nv,
+                     out, reinterpret_cast<unsigned long long*>(total), ctr, by_in ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -883,32 +931,14 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
   return hipGetLastError();
 }
 
-// Rows of the orientation (mode 3, out-neighbours, into ooff/oind) or of
-// its reverse (mode 4, in-neighbours, into roff/rind).
-hipError_t launch_orient_rows(const Prep& p, int64_t nv, const int32_t* ind, int mode, int64_t* roff, int32_t* rind,
-                              int64_t cap, int cus, hipStream_t s) {
-  const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 16);
-  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
-  hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
-  if (e != hipSuccess) return e;
-  if (nv > 0)
-    hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr,
-                       nv, mode, (const int32_t*)nullptr, p.ecnt, ctr);
-  e = scan_excl<int32_t>(p.ecnt, nullptr, nv, roff, p.scan_tmp, s);
-  if (e != hipSuccess) return e;
-  if (nv > 0)
-    hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, (const int64_t*)p.voff, ind, (const int32_t*)nullptr, nv,
-                       mode, (const int32_t*)nullptr, (const int64_t*)roff, cap, (int32_t*)nullptr, rind,
-                       (int32_t*)nullptr, ctr + 1);
-  // fill wrote only positions below cap; rows are cut to match
-  hipLaunchKernelGGL(k_clamp_i64, dim3(grid_for(nv + 1, 256, 4096)), dim3(256), 0, s, roff, nv + 1, cap);
-  return hipGetLastError();
-}
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
-  return launch_orient_rows(p, nv, ind, 3, p.ooff, p.oind, p.ocap, cus, s);
-}
-hipError_t launch_orient_in(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
-  return launch_orient_rows(p, nv, ind, 4, p.ioff, p.iind, p.ocap, cus, s);
+  if (!p.orow || !p.oind) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_i64(p.dscal + 5, 1, 0, s);
+  if (e != hipSuccess || nv <= 0) return e;
+  hipLaunchKernelGGL(k_orient_rows, dim3(grid_for(nv * 64, 256, (int64_t)cus * 16)), dim3(256), 0, s,
+                     (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
+                     reinterpret_cast<unsigned long long*>(p.dscal + 5));
+  return hipGetLastError();
 }
 
 int occupancy(void (*fn)(CountArgs), int bs) {

@@ -6,16 +6,19 @@
 // the arc a -> b.  Crediting 2 to each of a, b and c per triangle gives
 // out[v] = 2 T(v), the reference's per-vertex value, and the total 6 T.
 //
-// Every row r has partners p from toff/tind: either its own out-neighbours
-// (toff/tind = ooff/oind, arcs r -> p) or its in-neighbours (toff/tind =
-// the reversed orientation, arcs p -> r).  Every arc is the pair (r, p) of
-// exactly one row either way, and the arc's matches are N+(r) ∩ N+(p).  The
-// second form streams N+ of the lower end of each arc, which under degree
-// order is the shorter list on average, but a hub row then has as many
-// partners as the hub has lower neighbours.  Work is therefore handed out
-// in pieces of P consecutive partner arcs (counter ctr); prow[b] is the row
-// holding arc b * P, and a piece walks the rows it overlaps, so a hub row
-// spreads over many waves and a run of short rows shares one.
+// The orientation lives in place of the input rows (k_orient_rows): row v
+// keeps the positions voff[v] .. voff[v+1] of oind, N+(v) first (orow[2v]
+// .. orow[2v+1], ascending), then v's in-neighbours.  Every row r has
+// partners p: either its out-neighbours (arcs r -> p) or, with by_in, its
+// in-neighbours (arcs p -> r).  Every arc is the pair (r, p) of exactly one
+// row either way, and the arc's matches are N+(r) ∩ N+(p).  The second form
+// streams N+ of the lower end of each arc, on average the shorter list
+// under degree order, but a hub row then has as many partners as the hub
+// has lower neighbours.  Work is therefore handed out in pieces of P
+// consecutive arc positions (counter ctr); prow[b] is the row holding
+// position b * P, and a piece walks the rows it overlaps, taking the
+// partners that fall inside it, so a hub row spreads over many waves and a
+// run of short rows shares one.
 //
 // N+(r) is staged in the wave's LDS slice A next to 16-bit counters X (two
 // to a word), and for every partner p the wave streams N+(p) against A
@@ -36,6 +39,8 @@
 // short N+(p) (the usual case) keep every lane busy.  Groups may hit the
 // same A position; X is updated with LDS atomics.
 //
+// Without PV (the total only) there is no X and no per-vertex credit.
+//
 // Occupancy decides this kernel's speed (it waits on memory and LDS, and
 // the time falls in proportion to the waves per SIMD), so the lane's merge
 // is kept small: 32-bit chunk and element offsets from a 64-byte aligned
@@ -46,7 +51,7 @@
 // masked out.  A: the staged window (n elements, A[n] = INT_MAX).  Every
 // match at A position p adds 2 to X's 16-bit counter p; returns the
 // matches.
-template <int CH, bool PF>
+template <int CH, bool PF, bool XC>
 __device__ __forceinline__ int32_t orient_stream(const int32_t* __restrict__ bp, int32_t c0, int32_t c1, int32_t e0,
                                                  int32_t e1, const int32_t* A, int32_t n, int32_t* X) {
   constexpr int Q = CH / 4;
@@ -96,32 +101,32 @@ __device__ __forceinline__ int32_t orient_stream(const int32_t* __restrict__ bp,
       while (acur < bv) acur = A[++pos];
       if (acur == bv) {
         ++cnt;
-        x_credit(X, pos);
+        if (XC) x_credit(X, pos);
       }
     }
   }
   return cnt;
 }
 
-template <int AW, int G, int P, bool PF, int WPE>
+template <int AW, int G, int P, bool PF, int WPE, bool PV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
-    const int64_t* __restrict__ ooff, const int32_t* __restrict__ oind, const int64_t* __restrict__ toff,
-    const int32_t* __restrict__ tind, const int32_t* __restrict__ prow, int64_t nv, int32_t* __restrict__ out,
-    unsigned long long* total, unsigned long long* ctr) {
+    const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
+    const int32_t* __restrict__ prow, int64_t nv, int32_t* __restrict__ out, unsigned long long* total,
+    unsigned long long* ctr, int by_in) {
   constexpr int NW = 4;
   constexpr int CH = 16;
   constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
   static_assert(2 * P < 65536, "a piece's counts must fit 16 bits");
   __shared__ int32_t abuf_all[NW * (AW + 1)];
-  __shared__ int32_t xcnt_all[NW * XW];
+  __shared__ int32_t xcnt_all[PV ? NW * XW : 1];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int32_t* A = abuf_all + wv * (AW + 1);
-  int32_t* X = xcnt_all + wv * XW;
+  int32_t* X = xcnt_all + (PV ? wv * XW : 0);
   const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(oind) >> 2) & (CH - 1));
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
-  const int64_t M = toff[nv];  // partner arcs
+  const int64_t M = voff[nv];  // arc positions
   for (;;) {
     int64_t b = 0;
     if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
@@ -130,15 +135,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     if (a0 >= M) break;
     const int64_t a1 = a0 + P < M ? a0 + P : M;
     // rows overlapping [a0, a1): from prow[b] until one ends at or past a1
-    // (toff[nv] = M >= a1 stops the walk at row nv - 1 at the latest)
+    // (voff[nv] = M >= a1 stops the walk at row nv - 1 at the latest)
     int64_t u = prow[b];
-    int64_t t_next = toff[u];
+    int64_t t_next = voff[u];
     for (;; ++u) {
       const int64_t t_lo = t_next;
-      t_next = toff[u + 1];
-      const int64_t tu = t_lo > a0 ? t_lo : a0;
-      const int64_t lt = (t_next < a1 ? t_next : a1) - tu;  // partners of row u in this piece
-      const int64_t ou = ooff[u], lu = ooff[u + 1] - ou;
+      t_next = voff[u + 1];
+      const int64_t ou = orow[2 * u], oe = orow[2 * u + 1], lu = oe - ou;  // N+(u)
+      // partners: out-neighbours at [t_lo, oe), or in-neighbours at [oe, t_next)
+      const int64_t p0 = by_in ? oe : t_lo, p1 = by_in ? t_next : oe;
+      const int64_t tu = p0 > a0 ? p0 : a0;
+      const int64_t lt = (p1 < a1 ? p1 : a1) - tu;  // partners of row u in this piece
       if (lu == 0 || lt <= 0) {
         if (t_next >= a1) break;
         continue;
@@ -147,7 +154,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       for (int64_t ws = 0; ws < lu; ws += AW) {
         const int32_t n = (int32_t)(lu - ws < AW ? lu - ws : AW);
         for (int32_t y = lane; y < n; y += 64) A[y] = oind[ou + ws + y];
-        for (int32_t y = lane; y < (n + 1) / 2; y += 64) X[y] = 0;
+        if (PV)
+          for (int32_t y = lane; y < (n + 1) / 2; y += 64) X[y] = 0;
         if (lane == 0) A[n] = INT_MAX;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -162,17 +170,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             const int64_t jj = j + gid;
             int32_t c = 0;
             if (jj < lt) {
-              const int32_t w = tind[tu + jj];
-              const int64_t ow = ooff[w], we = ooff[w + 1];
+              const int32_t w = oind[tu + jj];
+              const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
               if (we > ow) {
                 const int64_t cb = (ow + sh) / CH;  // first chunk of N+(w) in ind4
                 const int32_t e0 = (int32_t)(ow + sh - cb * CH), e1 = (int32_t)(we + sh - cb * CH);
                 const int32_t nc = (e1 + CH - 1) / CH;
-                c = orient_stream<CH, PF>(ind4 + cb * CH, (nc * gl) / LG, (nc * (gl + 1)) / LG, e0, e1, A, n, X);
+                c = orient_stream<CH, PF, PV>(ind4 + cb * CH, (nc * gl) / LG, (nc * (gl + 1)) / LG, e0, e1, A, n, X);
               }
 #pragma unroll
               for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-              if (gl == 0 && c) atomicAdd(&out[w], 2 * c);
+              if (PV && gl == 0 && c) atomicAdd(&out[w], 2 * c);
             }
             cg += c;
           }
@@ -182,15 +190,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
           cu += uniform64(rs);
         } else {
-          // row offsets of partner j as 32-bit words in lanes 0-3, one ahead
+          // (begin, end) of N+ of partner j as 32-bit words in lanes 0-3, one ahead
           auto off_load = [&](int64_t j) -> int32_t {
             if (j >= lt || lane >= 4) return 0;
-            const int32_t w = tind[tu + j];
-            return reinterpret_cast<const int32_t*>(ooff + w + (lane >> 1))[lane & 1];
+            const int32_t w = oind[tu + j];
+            return reinterpret_cast<const int32_t*>(orow + 2 * (int64_t)w)[lane];
           };
           int32_t offs = off_load(0);
           for (int64_t j = 0; j < lt; ++j) {
-            const int32_t w = tind[tu + j];
+            const int32_t w = oind[tu + j];
             const int64_t ow = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 0) |
                                ((int64_t)__builtin_amdgcn_readlane(offs, 1) << 32);
             const int64_t we = (int64_t)(uint32_t)__builtin_amdgcn_readlane(offs, 2) |
@@ -206,28 +214,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               const int64_t cb = g0 / CH;
               const int32_t e0 = (int32_t)(g0 - cb * CH), e1 = (int32_t)(g1 - cb * CH);
               const int32_t nc = (e1 + CH - 1) / CH;
-              c = orient_stream<CH, PF>(ind4 + cb * CH, (nc * lane) / 64, (nc * (lane + 1)) / 64, e0, e1, A, n, X);
+              c = orient_stream<CH, PF, PV>(ind4 + cb * CH, (nc * lane) / 64, (nc * (lane + 1)) / 64, e0, e1, A, n,
+                                            X);
             }
             c = wave_sum32(c);
             if (c) {
-              if (lane == 0) atomicAdd(&out[w], 2 * c);
+              if (PV && lane == 0) atomicAdd(&out[w], 2 * c);
               cu += c;
             }
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int32_t y = lane; y < n; y += 64) {
-          const int32_t x = x_read(X, y);
-          if (x) atomicAdd(&out[A[y]], x);
+        if (PV) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          for (int32_t y = lane; y < n; y += 64) {
+            const int32_t x = x_read(X, y);
+            if (x) atomicAdd(&out[A[y]], x);
+          }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();  // A and X are rewritten by the next window
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
       if (cu) {
-        if (lane == 0) atomicAdd(&out[u], (int32_t)(2 * cu));
+        if (PV && lane == 0) atomicAdd(&out[u], (int32_t)(2 * cu));
         tri += cu;
       }
       if (t_next >= a1) break;
