@@ -468,7 +468,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv_total, nnz_total, true, Scheme::kEdges);
+  const Layout L = layout(nv_total, nnz_total, true, scheme_of(cfg));
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c, L, true);
@@ -476,11 +476,23 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
   if ((e = tcd::launch_fill_i32(d_out_cat, nv_total, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_voff_batch(d_tab, G, d_off_cat, nv_total, p.voff, p.vbase, s)) != hipSuccess)
     return hip_fail(c, e, "voff");
-  if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
-    return hip_fail(c, e, "edge list");
-  record(c, 1, s);
-  rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
-  if (rc) return rc;
+  if (L.sc == Scheme::kOrient) {
+    // The batch is one disjoint graph in global ids: the orientation
+    // (graph-local ids translated through p.vbase) never joins two graphs,
+    // so every triangle stays inside its graph.
+    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    record(c, 1, s);
+    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, cfg.row_groups, cfg.oriented == 2, c->cus,
+                                      s)) != hipSuccess)
+      return hip_fail(c, e, "count kernel launch");
+  } else {
+    if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "edge list");
+    record(c, 1, s);
+    rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
+    if (rc) return rc;
+  }
   c->last_dscal = p.dscal;
   record(c, 2, s);
   if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
@@ -621,7 +633,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
-  const Scheme sc = batch ? Scheme::kEdges : scheme_of(cfg);
+  const Scheme sc = scheme_of(cfg);
   if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(cfg.row_groups, true, c->cus);
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
