@@ -63,15 +63,16 @@ typedef struct tc_ctx tc_ctx;
  *                  least probe_ratio times longer than A is merged from A's
  *                  side, each A value tested against a 16-element chunk of B
  *                  in registers (default 4; negative: never)
- *   oriented       tc_count_i32 with algo 0: 1 (default) = intersect the
- *                  degree-ordered orientation, one wave per oriented row,
- *                  each triangle found once and its three vertices credited;
- *                  2 = the same, but a wave takes the in-neighbours of its
- *                  row as partners (streams the lower end's row of each arc);
- *                  -1 = every undirected edge's full lists (the reference's
- *                  scheme).  Same counts either way; other algos always use
- *                  the full lists.  tc_last_edge_work reports (0, 0) for the
- *                  oriented scheme, which builds no edge list.
+ *   oriented       algo 0 (tc_count_i32, tc_batch_count_i32): 1 (default) =
+ *                  intersect the degree-ordered orientation, built in place
+ *                  of the input rows, each triangle found once and its three
+ *                  vertices credited; 2 = the same, but a row takes its
+ *                  in-neighbours as partners (streams the lower end's row of
+ *                  each arc); -1 = every undirected edge's full lists (the
+ *                  reference's scheme, the edge-list kernels).  Same counts
+ *                  either way; other algos always use the full lists.
+ *                  tc_last_edge_work reports (0, 0) for the oriented scheme,
+ *                  which builds no edge list.
  *   row_groups     oriented scheme: arcs of a row a wave streams at once
  *                  (1, 2, 4 or 8; default 4) */
 typedef struct tc_cfg {
