@@ -97,6 +97,42 @@ def test_one_way_arcs_stay_in_bounds(counter):
     assert int(counter.count_total(d_off, d_ind).item()) == 0
 
 
+def test_relabelling_invariance(counter):
+    """The orientation breaks degree ties by vertex id, so a relabelled
+    graph is oriented differently; the counts must follow the relabelling
+    exactly (out'[p(v)] = out[v]) on both schemes."""
+    rng = np.random.default_rng(11)
+    for g in [G.rmat(13, 16, seed=5), GR.complete(60), G.geometric(20000, 6.0, seed=2)]:
+        perm = rng.permutation(g.n).astype(np.int64)
+        d = np.diff(g.off.astype(np.int64))
+        src = np.repeat(np.arange(g.n, dtype=np.int64), d)
+        h = G.from_edges(g.n, perm[src], perm[g.ind.astype(np.int64)])
+        assert h.nnz == g.nnz
+        ref, rtot = _ref(g)
+        for cfg in (None, dict(oriented=2), dict(oriented=-1)):
+            print(f"case relabelled {g.name} cfg={cfg}", flush=True)
+            out, tot = counter.count_graph(h, cfg=cfg)
+            np.testing.assert_array_equal(out[perm], ref)
+            assert tot == rtot
+
+
+def test_full_size_configs(counter):
+    """The parity configurations at full size: R-MAT s20 ef16 and RGG
+    n=4,194,304 (mean degree 6) against the oracle (OpenMP), bit for bit,
+    on the oriented and the edge scheme."""
+    import os
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    for g in [G.geometric(4194304, 6.0, seed=1), G.rmat(20, 16, seed=1)]:
+        print(f"case full size {g.name} n={g.n} nnz={g.nnz}: oracle on {threads} threads", flush=True)
+        ref, rtot = oracle.per_vertex(g.off, g.ind, threads=threads)
+        for cfg in (None, dict(oriented=-1)):
+            print(f"case full size {g.name} cfg={cfg}", flush=True)
+            out, tot = counter.count_graph(g, cfg=cfg)
+            np.testing.assert_array_equal(out, ref)
+            assert tot == rtot
+
+
 def test_oriented_only_for_algo_0(counter):
     """Other algos count the full lists whatever cfg.oriented says."""
     g = G.rmat(11, 16, seed=1)
@@ -110,7 +146,7 @@ def test_oriented_only_for_algo_0(counter):
 def test_variants(counter, v, small_max, ranges):
     g = G.rmat(13, 16, seed=21)
     cfg = dict(algo=v[0], wg_size=v[1], items=v[2], ring=v[3], chunk=v[4], depth=v[5], small_max=small_max,
-               ranges_per_wg=ranges)
+               ranges_per_wg=ranges, oriented=-1)  # the edge-list kernels these variants size
     _check(counter, g, cfg)
 
 
@@ -134,7 +170,7 @@ def test_search_ratio(counter, algo, ratio, ranges):
     have a shorter list longer than one wave's LDS slice (algo 3 walks it
     in windows)."""
     for g in _search_graphs():
-        _check(counter, g, cfg=dict(algo=algo, search_ratio=ratio, ranges_per_wg=ranges))
+        _check(counter, g, cfg=dict(algo=algo, search_ratio=ratio, ranges_per_wg=ranges, oriented=-1))
 
 
 @pytest.mark.parametrize("algo", [0, 3])
@@ -144,7 +180,8 @@ def test_probe_ratio(counter, algo, probe, search):
     """Streamed edges with B at least probe times longer than A are merged
     from A's side; probe 1 sends every streamed edge there, -1 none."""
     for g in _search_graphs():
-        _check(counter, g, cfg=dict(algo=algo, probe_ratio=probe, search_ratio=search, ranges_per_wg=4096))
+        _check(counter, g, cfg=dict(algo=algo, probe_ratio=probe, search_ratio=search, ranges_per_wg=4096,
+                                    oriented=-1))
 
 
 def _list_expect(g, vl):
