@@ -54,7 +54,8 @@ def test_random_graphs(counter, g):
 _ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1, probe_ratio=-1),
              dict(oriented=1, row_groups=1), dict(oriented=1, row_groups=2), dict(oriented=1, row_groups=8),
              dict(oriented=1, row_groups=8, probe_ratio=1),
-             dict(oriented=2), dict(oriented=2, row_groups=1), dict(oriented=2, row_groups=8, probe_ratio=1)]
+             dict(oriented=2), dict(oriented=2, row_groups=1), dict(oriented=2, row_groups=8, probe_ratio=1),
+             dict(oriented=1, row_groups=4)]
 
 
 @pytest.mark.parametrize("cfg", _ORIENTED, ids=str)
@@ -114,6 +115,26 @@ def test_relabelling_invariance(counter):
             out, tot = counter.count_graph(h, cfg=cfg)
             np.testing.assert_array_equal(out[perm], ref)
             assert tot == rtot
+
+
+def test_tied_hubs(counter):
+    """Two hubs of equal degree 70001, beyond 16 bits, tie in the degree
+    order and are ordered by id.  K_2 joined to 70000 leaves (every leaf in
+    one triangle with both hubs): closed form hubs 2*70000, leaves 2, on
+    every scheme and in the global-count mode."""
+    k = 70000
+    leaves = np.arange(2, k + 2)
+    src = np.concatenate([[0], np.zeros(k, np.int64), np.ones(k, np.int64)])
+    dst = np.concatenate([[1], leaves, leaves])
+    g = G.from_edges(k + 2, src, dst, name=f"k2_join_{k}")
+    exp = np.full(k + 2, 2, np.int64)
+    exp[:2] = 2 * k
+    for cfg in (None, dict(oriented=2), dict(row_groups=1), dict(oriented=-1)):
+        out, tot = counter.count_graph(g, cfg=cfg)
+        np.testing.assert_array_equal(out, exp)
+        assert tot == 6 * k
+    off, ind = counter.to_device(g)
+    assert int(counter.count_total(off, ind).item()) == 6 * k
 
 
 def test_full_size_configs(counter):
