@@ -310,13 +310,58 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
   return dv < dw || (dv == dw && v < w);
 }
 
-// Orientation in place, one wave per vertex (chunks from ctr): the arcs of
-// row v are written back to oind at the row's own positions voff[v] ..
-// voff[v+1], the kept ones N+(v) from the front in ascending order, the
-// others (v's in-neighbours) from the back; orow[2v] = voff[v] and
-// orow[2v+1] = voff[v] + |N+(v)|.  One pass, no scan, and a row never
-// spills into another whatever the input (a CSR that is not symmetric
-// only orients differently).
+// Orientation in place.  The arcs of row v are written back to oind at the
+// row's own positions voff[v] .. voff[v+1], the kept ones N+(v) from the
+// front in ascending order, the others (v's in-neighbours) from the back;
+// orow[2v] = voff[v] and orow[2v+1] = voff[v] + |N+(v)|.  One pass, no
+// scan, and a row never spills into another whatever the input (a CSR that
+// is not symmetric only orients differently).
+//
+// Waves take 64 vertices at a time from ctr.  A row of at most kShortRow
+// arcs is done by its own lane, four arcs per step (sparse graphs have
+// rows of a few arcs, which would leave most of a wave idle); the longer
+// rows of the 64 are then done one at a time by the whole wave, 256 arcs
+// per step, with ballots placing the kept and the other arcs.
+constexpr int64_t kShortRow = 32;
+
+__device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+                                                int32_t* __restrict__ oind, int64_t* __restrict__ orow, int64_t v,
+                                                int64_t b, int64_t e, int32_t vb, int lane, uint64_t lt) {
+  const int64_t dv = e - b;
+  const int32_t vl = (int32_t)(v - vb);
+  int64_t pf = b, pb = e;  // next front position, one past the next back one
+  for (int64_t k0 = b; k0 < e; k0 += 256) {
+    int32_t wl[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t k = k0 + lane + 64 * r;
+      wl[r] = k < e ? ind[k] : -1;
+    }
+    int64_t lo[4], hi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
+      lo[r] = voff[wg];
+      hi[r] = voff[wg + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool valid = wl[r] >= 0;
+      const bool keep = valid && orient_keep(dv, vl, hi[r] - lo[r], wl[r]);
+      const uint64_t mk = __ballot(keep), mi = __ballot(valid && !keep);
+      const int32_t wg = (int32_t)((int64_t)vb + wl[r]);
+      if (keep) oind[pf + __popcll(mk & lt)] = wg;
+      else if (valid) oind[pb - 1 - __popcll(mi & lt)] = wg;
+      pf += __popcll(mk);
+      pb -= __popcll(mi);
+    }
+  }
+  if (lane == 0) {
+    orow[2 * v] = b;
+    orow[2 * v + 1] = pf;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
                                                      const int32_t* __restrict__ vbase, int64_t nv,
                                                      int32_t* __restrict__ oind, int64_t* __restrict__ orow,
@@ -324,44 +369,54 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (;;) {
-    const int64_t v0 = grab_vertices(ctr);
+    int64_t v0 = 0;
+    if (lane == 0) v0 = (int64_t)atomicAdd(ctr, 64ull);
+    v0 = uniform64(__shfl(v0, 0, 64));
     if (v0 >= nv) break;
-    const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
-    for (int64_t v = v0; v < v1; ++v) {
-      const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-      const int32_t vb = vbase ? vbase[v] : 0;
+    const int64_t v = v0 + lane;
+    const bool mine = v < nv;
+    int64_t b = 0, e = 0;
+    int32_t vb = 0;
+    if (mine) {
+      b = voff[v];
+      e = voff[v + 1];
+      vb = vbase ? vbase[v] : 0;
+    }
+    if (mine && e - b <= kShortRow) {
+      const int64_t dv = e - b;
       const int32_t vl = (int32_t)(v - vb);
-      int64_t pf = b, pb = e;  // next front position, one past the next back one
-      for (int64_t k0 = b; k0 < e; k0 += 256) {
+      int64_t pf = b, pb = e;
+      for (int64_t k0 = b; k0 < e; k0 += 4) {
         int32_t wl[4];
+        int64_t dw[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t k = k0 + lane + 64 * r;
-          wl[r] = k < e ? ind[k] : -1;
-        }
-        int64_t lo[4], hi[4];
+        for (int r = 0; r < 4; ++r) wl[r] = k0 + r < e ? ind[k0 + r] : -1;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
-          lo[r] = voff[wg];
-          hi[r] = voff[wg + 1];
+          dw[r] = voff[wg + 1] - voff[wg];
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const bool valid = wl[r] >= 0;
-          const bool keep = valid && orient_keep(dv, vl, hi[r] - lo[r], wl[r]);
-          const uint64_t mk = __ballot(keep), mi = __ballot(valid && !keep);
+          if (wl[r] < 0) continue;
           const int32_t wg = (int32_t)((int64_t)vb + wl[r]);
-          if (keep) oind[pf + __popcll(mk & lt)] = wg;
-          else if (valid) oind[pb - 1 - __popcll(mi & lt)] = wg;
-          pf += __popcll(mk);
-          pb -= __popcll(mi);
+          if (orient_keep(dv, vl, dw[r], wl[r])) oind[pf++] = wg;
+          else oind[--pb] = wg;
         }
       }
-      if (lane == 0) {
-        orow[2 * v] = b;
-        orow[2 * v + 1] = pf;
-      }
+      orow[2 * v] = b;
+      orow[2 * v + 1] = pf;
+    }
+    // the long rows among the 64, one at a time by the whole wave
+    uint64_t lm = __ballot(mine && e - b > kShortRow);
+    while (lm) {
+      const int l = __builtin_ctzll(lm);
+      lm &= lm - 1;
+      const int64_t bl = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(uint64_t)b, l) |
+                         ((int64_t)__builtin_amdgcn_readlane((int32_t)((uint64_t)b >> 32), l) << 32);
+      const int64_t el = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(uint64_t)e, l) |
+                         ((int64_t)__builtin_amdgcn_readlane((int32_t)((uint64_t)e >> 32), l) << 32);
+      orient_row_wave(voff, ind, oind, orow, v0 + l, bl, el, __builtin_amdgcn_readlane(vb, l), lane, lt);
     }
   }
 }
