@@ -53,7 +53,7 @@ tc_cfg builtin_default() {
   c.search_ratio = 64;
   c.probe_ratio = 4;
   c.oriented = 1;
-  c.row_groups = 4;
+  c.row_groups = -1;  // automatic (orient_groups)
   return c;
 }
 
@@ -84,12 +84,27 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (v.wg_size == c.wg_size && v.items == c.items && v.ring == c.ring && v.algo == want && stream_ok) vi = i;
   }
   if (vi < 0 || c.ranges_per_wg < 1 || c.ranges_per_wg > 4096 || c.small_max < 1 || c.algo < 0 || c.algo > 3) return -1;
-  if (c.row_groups != 1 && c.row_groups != 2 && c.row_groups != 4 && c.row_groups != 8) return -1;
+  if (c.row_groups > 0 && c.row_groups != 1 && c.row_groups != 2 && c.row_groups != 4 && c.row_groups != 8 &&
+      c.row_groups != 16 && c.row_groups != 32)
+    return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   if (c.algo != 0) c.oriented = 0;  // the oriented kernel is algo 0's
   *out = c;
   return vi;
+}
+
+// Partners a wave streams at once on the oriented scheme when row_groups
+// asks for the automatic choice (<= 0).  Sparser and smaller graphs have
+// shorter N+ lists, which want more groups of fewer lanes; measured on
+// R-MAT s24 (4 best), R-MAT s20 (8), the mixed 4096-graph batch (16) and
+// RGG n=4M (32).
+int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz) {
+  if (cfg.row_groups > 0) return cfg.row_groups;
+  const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
+  if (dbar < 10) return 32;
+  if (dbar < 20) return 16;
+  return nnz >= (int64_t(1) << 28) ? 4 : 8;
 }
 
 // Scratch layout of one call.  The edge-list scheme keeps an undirected
@@ -368,8 +383,8 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
-    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, cfg.row_groups, cfg.oriented == 2, c->cus, s)) !=
-        hipSuccess)
+    const int groups = orient_groups(cfg, nv, nnz);
+    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
@@ -408,7 +423,8 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
-  if ((e = tcd::launch_count_orient(p, nv, nullptr, d_total, cfg.row_groups, false, c->cus, s)) != hipSuccess)
+  const int groups = orient_groups(cfg, nv, nnz);
+  if ((e = tcd::launch_count_orient(p, nv, nullptr, d_total, groups, false, c->cus, s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -483,8 +499,9 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
-    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, cfg.row_groups, cfg.oriented == 2, c->cus,
-                                      s)) != hipSuccess)
+    const int groups = orient_groups(cfg, nv_total, nnz_total);
+    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s)) !=
+        hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
@@ -634,7 +651,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
   const Scheme sc = scheme_of(cfg);
-  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(cfg.row_groups, true, c->cus);
+  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz), true, c->cus);
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 

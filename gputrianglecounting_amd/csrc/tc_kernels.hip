@@ -937,14 +937,16 @@ OrientFn orient_fn_pv(int groups) {
     case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE, PV>;
     case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE, PV>;
     case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE, PV>;
+    case 16: return k_count_orient<kOrientAW, 16, P, kOrientPF, kOrientWPE, PV>;
+    case 32: return k_count_orient<kOrientAW, 32, P, kOrientPF, kOrientWPE, PV>;
     default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE, PV>;
   }
 }
 OrientFn orient_fn(int groups, bool pv) { return pv ? orient_fn_pv<true>(groups) : orient_fn_pv<false>(groups); }
 int resident_orient_wgs(int groups, bool pv, int cus) {
   static std::mutex mu;
-  static int occ[2][9] = {};
-  const int gi = groups == 1 || groups == 2 || groups == 8 ? groups : 4;
+  static int occ[2][33] = {};
+  const int gi = groups == 1 || groups == 2 || groups == 8 || groups == 16 || groups == 32 ? groups : 4;
   std::lock_guard<std::mutex> lk(mu);
   if (!occ[pv][gi]) {
     int n = 0;

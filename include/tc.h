@@ -73,8 +73,11 @@ typedef struct tc_ctx tc_ctx;
  *                  either way; other algos always use the full lists.
  *                  tc_last_edge_work reports (0, 0) for the oriented scheme,
  *                  which builds no edge list.
- *   row_groups     oriented scheme: arcs of a row a wave streams at once
- *                  (1, 2, 4 or 8; default 4) */
+ *   row_groups     oriented scheme: partners of a row a wave streams at once,
+ *                  each by 64/row_groups lanes (1, 2, 4, 8, 16 or 32;
+ *                  default -1 = chosen from the mean degree and size:
+ *                  32 below mean degree 10, 16 below 20, else 4 from 2^28
+ *                  arcs up and 8 below) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;

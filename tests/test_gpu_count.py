@@ -55,7 +55,8 @@ _ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1,
              dict(oriented=1, row_groups=1), dict(oriented=1, row_groups=2), dict(oriented=1, row_groups=8),
              dict(oriented=1, row_groups=8, probe_ratio=1),
              dict(oriented=2), dict(oriented=2, row_groups=1), dict(oriented=2, row_groups=8, probe_ratio=1),
-             dict(oriented=1, row_groups=4)]
+             dict(oriented=1, row_groups=4), dict(oriented=1, row_groups=16), dict(oriented=1, row_groups=32),
+             dict(oriented=2, row_groups=32)]
 
 
 @pytest.mark.parametrize("cfg", _ORIENTED, ids=str)
