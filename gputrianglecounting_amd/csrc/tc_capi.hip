@@ -659,8 +659,17 @@ struct Candidate {
   tc_cfg cfg;
 };
 
+// The oriented scheme at every partners-per-wave setting, then the edge
+// scheme's streamed kernels (algo 0 and 3) at each compiled size.
 std::vector<tc_cfg> tune_candidates() {
   std::vector<tc_cfg> out;
+  for (int g : {2, 4, 8, 16, 32}) {
+    tc_cfg c{};
+    c.algo = 0;
+    c.oriented = 1;
+    c.row_groups = g;
+    out.push_back(c);
+  }
   for (int i = 0; i < tcd::num_variants(); ++i) {
     const tcd::Variant v = tcd::variant(i);
     if (v.algo != 0 && v.algo != 3) continue;  // the tuner searches the streamed kernels
@@ -678,6 +687,7 @@ std::vector<tc_cfg> tune_candidates() {
         c.algo = v.algo;
         c.chunk = v.chunk;
         c.depth = v.depth;
+        c.oriented = -1;
         out.push_back(c);
       }
     }

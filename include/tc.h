@@ -209,8 +209,10 @@ int tc_graph_launch(tc_graph* g, void* stream);
 int tc_graph_destroy(tc_graph* g);
 
 /* ---------------------------------------------------------------- autotuner
- * Candidates: every compiled kernel variant (wg_size, items, ring) times
- * small_max in {64, 256, 1024} (at most wg_size*items) times ranges_per_wg in
+ * Candidates: the oriented scheme with row_groups in {2, 4, 8, 16, 32},
+ * then the edge scheme (oriented = -1) at every compiled streamed kernel
+ * variant (wg_size, items, ring; algo 0 and 3) times small_max in
+ * {64, 256, 1024} (at most wg_size*items) times ranges_per_wg in
  * {8, 32, 128}.  Each candidate runs once, its per-vertex output is
  * compared with the first candidate's (identical must hold: the counts are
  * exact integers), then it is timed over reps launches with HIP events.
