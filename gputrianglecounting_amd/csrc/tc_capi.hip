@@ -400,6 +400,39 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   return TC_OK;
 }
 
+int tc_count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind, int64_t* d_out,
+                 int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || (nv > 0 && !d_out))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nv >= INT32_MAX) return set_err(c, TC_ERANGE, "more than 2^31 - 1 vertices");
+  tc_cfg cfg;
+  if (resolve(c, cfg_in, &cfg) < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  if (!cfg.oriented) return set_err(c, TC_EINVAL, "the int64 path runs the oriented scheme (algo 0, oriented >= 1)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv, nnz, false, Scheme::kOrient);
+  int rc = ensure(c, L.total);
+  if (rc) return rc;
+  tcd::Prep p = make_prep(c, L, false);
+  p.voff = const_cast<int64_t*>(d_off);  // the caller's offsets are already int64; the kernels only read them
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+  record(c, 1, s);
+  const int groups = orient_groups(cfg, nv, nnz);
+  if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
+    return hip_fail(c, e, "count kernel launch");
+  c->last_dscal = p.dscal;
+  record(c, 2, s);
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
 int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
                        int64_t* d_total, void* stream) {
   if (!c) return TC_EINVAL;
@@ -424,7 +457,8 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, nullptr, d_total, groups, false, c->cus, s)) != hipSuccess)
+  if ((e = tcd::launch_count_orient(p, nv, static_cast<int32_t*>(nullptr), d_total, groups, false, c->cus, s)) !=
+      hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -563,7 +597,7 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
   bool bs_ok = std::find(std::begin(kBlockSizes), std::end(kBlockSizes), numberThreads) != std::end(kBlockSizes);
   bool tpi_ok = std::find(std::begin(kTpi), std::end(kTpi), threads_per_block) != std::end(kTpi);
   if (!bs_ok || !tpi_ok || numberBlocks == 0 || number_blocks != (int32_t)numberThreads / threads_per_block ||
-      (1 << shifter) != threads_per_block || nv < 0 || !d_off)
+      shifter < 0 || shifter > 5 || (1 << shifter) != threads_per_block || nv < 0 || !d_off)
     return TC_EINVAL;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TC_EHIP;
@@ -580,6 +614,33 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
   hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return TC_EHIP;
   return tc_count_i32(c, nv, nnz, d_off, d_ind, outPutTriangles, nullptr, nullptr, nullptr);
+}
+
+
+int tc_kernelCall_i64(unsigned numberBlocks, unsigned numberThreads, int64_t nv, const int64_t* d_off,
+                      const int64_t* d_ind, int64_t* outPutTriangles, int64_t threads_per_block, int64_t number_blocks,
+                      int64_t shifter) {
+  static const unsigned kBlockSizes[] = {32, 64, 96, 128, 192, 256, 512, 1024};
+  static const int64_t kTpi[] = {1, 2, 4, 8, 16, 32};
+  bool bs_ok = std::find(std::begin(kBlockSizes), std::end(kBlockSizes), numberThreads) != std::end(kBlockSizes);
+  bool tpi_ok = std::find(std::begin(kTpi), std::end(kTpi), threads_per_block) != std::end(kTpi);
+  if (!bs_ok || !tpi_ok || numberBlocks == 0 || number_blocks != (int64_t)numberThreads / threads_per_block ||
+      shifter < 0 || shifter > 5 || (int64_t(1) << shifter) != threads_per_block || nv < 0 || !d_off)
+    return TC_EINVAL;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TC_EHIP;
+  tc_ctx* c;
+  {
+    std::lock_guard<std::mutex> lk(g_shim_mu);
+    if (!g_shim_ctx[dev]) {
+      if (tc_ctx_create(dev, &g_shim_ctx[dev]) != TC_OK) return TC_EHIP;
+    }
+    c = g_shim_ctx[dev];
+  }
+  int64_t nnz = 0;
+  hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return TC_EHIP;
+  return tc_count_i64(c, nv, nnz, d_off, d_ind, outPutTriangles, nullptr, nullptr, nullptr);
 }
 
 }  // extern "C"
@@ -651,7 +712,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
   const Scheme sc = scheme_of(cfg);
-  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz), true, c->cus);
+  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz), 1, c->cus);
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 

@@ -81,12 +81,14 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
 // Orientation in place (k_orient_rows) into p.orow / p.oind; ind holds
 // graph-local ids when p.vbase is set.
 hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
+hipError_t launch_orient(const Prep& p, int64_t nv, const int64_t* ind, int cus, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
 int resident_wgs2(int variant_idx, int cus, int which);
-// Resident workgroups of the oriented count kernel (per-vertex or total only).
-int resident_orient_wgs(int groups, bool pv, int cus);
+// Resident workgroups of the oriented count kernel; kind 0 = the total
+// only, 1 = int32 counts, 2 = int64 counts.
+int resident_orient_wgs(int groups, int kind, int cus);
 // nwg, nwg2, nwg3: grids of the streamed kernel's wave launch, its
 // workgroup launch and the lookup kernel (persistent: resident workgroups).
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
@@ -99,6 +101,8 @@ hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int3
 // arc (partners are its in-neighbours) instead of the lower one (partners
 // are its out-neighbours).
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
+                               int cus, hipStream_t s);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
                                int cus, hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);

@@ -324,7 +324,8 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
 // per step, with ballots placing the kept and the other arcs.
 constexpr int64_t kShortRow = 32;
 
-__device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+template <typename IT>
+__device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff, const IT* __restrict__ ind,
                                                 int32_t* __restrict__ oind, int64_t* __restrict__ orow, int64_t v,
                                                 int64_t b, int64_t e, int32_t vb, int lane, uint64_t lt) {
   const int64_t dv = e - b;
@@ -335,7 +336,7 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t k = k0 + lane + 64 * r;
-      wl[r] = k < e ? ind[k] : -1;
+      wl[r] = k < e ? (int32_t)ind[k] : -1;
     }
     int64_t lo[4], hi[4];
 #pragma unroll
@@ -362,7 +363,9 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
   }
 }
 
-__global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
+// IT: the caller's id type (int32 or int64 CSR); ids are below 2^31.
+template <typename IT>
+__global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const IT* __restrict__ ind,
                                                      const int32_t* __restrict__ vbase, int64_t nv,
                                                      int32_t* __restrict__ oind, int64_t* __restrict__ orow,
                                                      unsigned long long* ctr) {
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
         int32_t wl[4];
         int64_t dw[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) wl[r] = k0 + r < e ? ind[k0 + r] : -1;
+        for (int r = 0; r < 4; ++r) wl[r] = k0 + r < e ? (int32_t)ind[k0 + r] : -1;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
@@ -927,52 +930,67 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
 constexpr int kOrientAW = 851;
 constexpr bool kOrientPF = false;
 constexpr int kOrientWPE = 8;
-using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, int32_t*,
+template <typename OT>
+using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
                           unsigned long long*, unsigned long long*, int);
-// arcs per wave at a time (groups), per-vertex or total only -> kernel
-template <bool PV>
-OrientFn orient_fn_pv(int groups) {
+// partners per wave (groups), per-vertex output type (or the total only) -> kernel
+template <bool PV, typename OT>
+OrientFn<OT> orient_fn(int groups) {
   constexpr int P = (int)kOrientPiece;
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE, PV>;
-    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE, PV>;
-    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE, PV>;
-    case 16: return k_count_orient<kOrientAW, 16, P, kOrientPF, kOrientWPE, PV>;
-    case 32: return k_count_orient<kOrientAW, 32, P, kOrientPF, kOrientWPE, PV>;
-    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE, PV>;
+    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE, PV, OT>;
+    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE, PV, OT>;
+    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE, PV, OT>;
+    case 16: return k_count_orient<kOrientAW, 16, P, kOrientPF, kOrientWPE, PV, OT>;
+    case 32: return k_count_orient<kOrientAW, 32, P, kOrientPF, kOrientWPE, PV, OT>;
+    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE, PV, OT>;
   }
 }
-OrientFn orient_fn(int groups, bool pv) { return pv ? orient_fn_pv<true>(groups) : orient_fn_pv<false>(groups); }
-int resident_orient_wgs(int groups, bool pv, int cus) {
+// kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts
+const void* orient_kernel(int groups, int kind) {
+  if (kind == 1) return reinterpret_cast<const void*>(orient_fn<true, int32_t>(groups));
+  if (kind == 2) return reinterpret_cast<const void*>(orient_fn<true, int64_t>(groups));
+  return reinterpret_cast<const void*>(orient_fn<false, int32_t>(groups));
+}
+int resident_orient_wgs(int groups, int kind, int cus) {
   static std::mutex mu;
-  static int occ[2][33] = {};
+  static int occ[3][33] = {};
   const int gi = groups == 1 || groups == 2 || groups == 8 || groups == 16 || groups == 32 ? groups : 4;
+  const int ki = kind == 1 || kind == 2 ? kind : 0;
   std::lock_guard<std::mutex> lk(mu);
-  if (!occ[pv][gi]) {
+  if (!occ[ki][gi]) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(orient_fn(gi, pv)), 256, 0) !=
-            hipSuccess ||
-        n < 1)
-      n = 1;
-    occ[pv][gi] = n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, orient_kernel(gi, ki), 256, 0) != hipSuccess || n < 1) n = 1;
+    occ[ki][gi] = n;
   }
-  return occ[pv][gi] * cus;
+  return occ[ki][gi] * cus;
 }
 
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s) {
+template <bool PV, typename OT>
+hipError_t launch_count_orient_t(const Prep& p, int64_t nv, OT* out, int64_t* total, int groups, bool by_in, int kind,
+                                 int cus, hipStream_t s) {
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
-  const bool pv = out != nullptr;
   hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
                      kOrientPiece, p.prow);
-  hipLaunchKernelGGL(orient_fn(groups, pv), dim3(resident_orient_wgs(groups, pv, cus)), dim3(256), 0, s,
+  const OrientFn<OT> fn = orient_fn<PV, OT>(groups);
+  hipLaunchKernelGGL(fn, dim3(resident_orient_wgs(groups, kind, cus)), dim3(256), 0, s,
                      (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.prow, nv,
                      out, reinterpret_cast<unsigned long long*>(total), ctr, by_in ? 1 : 0);
   return hipGetLastError();
+}
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
+                               int cus, hipStream_t s) {
+  return out ? launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 1, cus, s)
+             : launch_count_orient_t<false>(p, nv, out, total, groups, by_in, 0, cus, s);
+}
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
+                               int cus, hipStream_t s) {
+  if (!out) return launch_count_orient_t<false>(p, nv, (int32_t*)nullptr, total, groups, by_in, 0, cus, s);
+  return launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 2, cus, s);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {
@@ -988,14 +1006,21 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
   return hipGetLastError();
 }
 
-hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
+template <typename IT>
+hipError_t launch_orient_t(const Prep& p, int64_t nv, const IT* ind, int cus, hipStream_t s) {
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
   hipError_t e = launch_fill_i64(p.dscal + 5, 1, 0, s);
   if (e != hipSuccess || nv <= 0) return e;
-  hipLaunchKernelGGL(k_orient_rows, dim3(grid_for(nv * 64, 256, (int64_t)cus * 16)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv * 64, 256, (int64_t)cus * 16)), dim3(256), 0, s,
                      (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
                      reinterpret_cast<unsigned long long*>(p.dscal + 5));
   return hipGetLastError();
+}
+hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
+  return launch_orient_t(p, nv, ind, cus, s);
+}
+hipError_t launch_orient(const Prep& p, int64_t nv, const int64_t* ind, int cus, hipStream_t s) {
+  return launch_orient_t(p, nv, ind, cus, s);
 }
 
 int occupancy(void (*fn)(CountArgs), int bs) {

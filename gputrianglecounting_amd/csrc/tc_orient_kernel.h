@@ -108,10 +108,17 @@ __device__ __forceinline__ int32_t orient_stream(const int32_t* __restrict__ bp,
   return cnt;
 }
 
-template <int AW, int G, int P, bool PF, int WPE, bool PV>
+// Per-vertex credit into the caller's output type (int32, or int64 for an
+// int64 CSR); v >= 0.
+__device__ __forceinline__ void out_add(int32_t* out, int64_t i, int64_t v) { atomicAdd(out + i, (int32_t)v); }
+__device__ __forceinline__ void out_add(int64_t* out, int64_t i, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(out + i), (unsigned long long)v);
+}
+
+template <int AW, int G, int P, bool PF, int WPE, bool PV, typename OT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
-    const int32_t* __restrict__ prow, int64_t nv, int32_t* __restrict__ out, unsigned long long* total,
+    const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
     unsigned long long* ctr, int by_in) {
   constexpr int NW = 4;
   constexpr int CH = 16;
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               }
 #pragma unroll
               for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-              if (PV && gl == 0 && c) atomicAdd(&out[w], 2 * c);
+              if (PV && gl == 0 && c) out_add(out, w, 2 * c);
             }
             cg += c;
           }
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
             }
             c = wave_sum32(c);
             if (c) {
-              if (PV && lane == 0) atomicAdd(&out[w], 2 * c);
+              if (PV && lane == 0) out_add(out, w, 2 * c);
               cu += c;
             }
           }
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
           for (int32_t y = lane; y < n; y += 64) {
             const int32_t x = x_read(X, y);
-            if (x) atomicAdd(&out[A[y]], x);
+            if (x) out_add(out, A[y], x);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
       if (cu) {
-        if (PV && lane == 0) atomicAdd(&out[u], (int32_t)(2 * cu));
+        if (PV && lane == 0) out_add(out, u, 2 * cu);
         tri += cu;
       }
       if (t_next >= a1) break;

@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
         "tc_last_timing": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "tc_count_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_kernelCall_i32": (C.c_int, [C.c_uint, C.c_uint, i32, vp, vp, vp, i32, i32, i32]),
+        "tc_count_i64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_kernelCall_i64": (C.c_int, [C.c_uint, C.c_uint, i64, vp, vp, vp, i64, i64, i64]),
         "tc_lcc_f64": (C.c_int, [vp, i32, vp, vp, vp, vp]),
         "tc_validate_i32": (C.c_int, [vp, i32, i64, vp, vp, C.POINTER(C.c_int), vp]),
         "tc_batch_count_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
@@ -132,7 +134,8 @@ def lib() -> C.CDLL:
 # Every symbol include/tc.h declares (checked by the CPU test suite).
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
-           "tc_count_i32", "tc_kernelCall_i32", "tc_lcc_f64", "tc_validate_i32", "tc_batch_count_i32",
+           "tc_count_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_lcc_f64", "tc_validate_i32",
+           "tc_batch_count_i32",
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
            "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
@@ -236,14 +239,17 @@ class TriangleCounter:
 
     # ------------------------------------------------------------ device API
     def count(self, off, ind, out=None, total=None, cfg=None, stream=None):
-        """Per-vertex counts of a device CSR (int32 torch tensors).
+        """Per-vertex counts of a device CSR (int32 torch tensors; an int64
+        pair goes to :meth:`count64`).
 
         Returns ``(out, total)``: ``out`` int32[nv] = 2·T(v), ``total`` int64[1]."""
         torch = self._torch
+        if off.dtype == torch.int64 and ind.dtype == torch.int64:
+            return self.count64(off, ind, out, total, cfg, stream)
         nv = int(off.shape[0]) - 1
         nnz = int(ind.shape[0])
         if off.dtype != torch.int32 or ind.dtype != torch.int32:
-            raise TypeError("off and ind must be int32")
+            raise TypeError("off and ind must both be int32 or both int64")
         if out is None:
             out = torch.empty(max(nv, 0), dtype=torch.int32, device=off.device)
         if total is None:
@@ -251,6 +257,24 @@ class TriangleCounter:
         rc = lib().tc_count_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(out) if nv else None,
                                 _p(total), _cfgp(cfg), _stream(stream))
         self._check(rc, "tc_count_i32")
+        return out, total
+
+    def count64(self, off, ind, out=None, total=None, cfg=None, stream=None):
+        """Per-vertex counts of an int64 device CSR (tc_count_i64, the
+        reference's kernelCall<int64_t>).  Returns ``(out, total)`` with
+        ``out`` int64[nv] = 2·T(v)."""
+        torch = self._torch
+        if off.dtype != torch.int64 or ind.dtype != torch.int64:
+            raise TypeError("off and ind must be int64")
+        nv = int(off.shape[0]) - 1
+        nnz = int(ind.shape[0])
+        if out is None:
+            out = torch.empty(max(nv, 0), dtype=torch.int64, device=off.device)
+        if total is None:
+            total = torch.empty(1, dtype=torch.int64, device=off.device)
+        rc = lib().tc_count_i64(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(out) if nv else None,
+                                _p(total), _cfgp(cfg), _stream(stream))
+        self._check(rc, "tc_count_i64")
         return out, total
 
     def count_total(self, off, ind, total=None, stream=None):
@@ -390,6 +414,13 @@ class TriangleCounter:
         out, totals = self.batch_count(tab, off, ind, b.total_vertices, b.total_arcs, cfg=cfg)
         self._torch.cuda.synchronize(self.device)
         return out.cpu().numpy(), totals.cpu().numpy()
+
+
+def kernel_call64(numberBlocks: int, numberThreads: int, nv: int, d_off, d_ind, out, threads_per_block: int,
+                  number_blocks: int, shifter: int) -> int:
+    """``kernelCall<int64_t>`` (clusteringCount.cu:518-524) with the reference argument list."""
+    return int(lib().tc_kernelCall_i64(numberBlocks, numberThreads, nv, _p(d_off), _p(d_ind), _p(out),
+                                       threads_per_block, number_blocks, shifter))
 
 
 def kernel_call(numberBlocks: int, numberThreads: int, nv: int, d_off, d_ind, out, threads_per_block: int,

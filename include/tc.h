@@ -144,6 +144,23 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
                       const int32_t* d_ind, int32_t* outPutTriangles, int32_t threads_per_block,
                       int32_t number_blocks, int32_t shifter);
 
+/* The int64 CSR (the reference's kernelCall<int64_t>, instantiated at
+ * src/clusteringCount.cu:518-524 and run by main, src/main.cu:36):
+ *   d_off[nv+1], d_ind[nnz]  device int64 CSR, rows sorted ascending; nnz
+ *                            may exceed 2^31, vertex ids must be below 2^31
+ *   d_out[nv]                device int64, fully overwritten: 2·T(v)
+ *   d_total                  device int64 or NULL: sum of d_out (= 6·T)
+ * Runs the oriented scheme (TC_EINVAL if cfg asks for another); the
+ * offsets are read in place, the ids once into the orientation. */
+int tc_count_i64(tc_ctx* ctx, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind,
+                 int64_t* d_out, int64_t* d_total, const tc_cfg* cfg, void* stream);
+
+/* Shape-compatible stand-in for kernelCall<int64_t>: the same argument
+ * rules as tc_kernelCall_i32, on tc_count_i64. */
+int tc_kernelCall_i64(unsigned numberBlocks, unsigned numberThreads, int64_t nv, const int64_t* d_off,
+                      const int64_t* d_ind, int64_t* outPutTriangles, int64_t threads_per_block,
+                      int64_t number_blocks, int64_t shifter);
+
 /* Vertex-list count: the reference's vertex-list overload of kernelCall<T>
  * (src/clusteringCount.cu:361-413, kernel :266-312; CPU form
  * src/clusteringCount.cpp:125-155).  d_list[nlist] holds vertex ids of the

@@ -316,6 +316,62 @@ def test_kernel_call_shim(counter):
     assert T.kernel_call(64, 256, g.n, off, ind, out, 4, 32, 2) == T.TC_EINVAL
 
 
+def _count64(counter, g, cfg=None):
+    import torch
+
+    off = torch.from_numpy(g.off.astype(np.int64)).cuda()
+    ind = torch.from_numpy(g.ind.astype(np.int64)).cuda()
+    out, tot = counter.count(off, ind, cfg=cfg)
+    torch.cuda.synchronize()
+    assert out.dtype == torch.int64
+    return out.cpu().numpy(), int(tot.item())
+
+
+@pytest.mark.parametrize("cfg", [None, dict(row_groups=1), dict(row_groups=32), dict(oriented=2)], ids=str)
+def test_int64_csr(counter, cfg):
+    """The int64 CSR path (kernelCall<int64_t>, the reference main's type):
+    int64 counts identical to the oracle on closed forms, random graphs and
+    the long-row and tied-hub cases."""
+    k = 70000
+    leaves = np.arange(2, k + 2)
+    tied = G.from_edges(k + 2, np.concatenate([[0], np.zeros(k, np.int64), np.ones(k, np.int64)]),
+                        np.concatenate([[1], leaves, leaves]), name=f"k2_join_{k}")
+    for g in GR.small_cases() + GR.random_cases() + [GR.complete(1800), tied]:
+        print(f"case int64 {g.name} n={g.n} nnz={g.nnz} cfg={cfg}", flush=True)
+        out, tot = _count64(counter, g, cfg)
+        ref, rtot = _ref(g)
+        np.testing.assert_array_equal(out, ref)
+        assert tot == rtot
+
+
+def test_int64_errors(counter):
+    import torch
+
+    g = G.rmat(10, 8, seed=2)
+    off = torch.from_numpy(g.off.astype(np.int64)).cuda()
+    ind = torch.from_numpy(g.ind.astype(np.int64)).cuda()
+    for bad in (dict(oriented=-1), dict(algo=3)):
+        with pytest.raises(T.TCError):
+            counter.count(off, ind, cfg=bad)
+    with pytest.raises(TypeError):
+        counter.count(off, ind.to(torch.int32))
+
+
+def test_kernel_call_shim_int64(counter):
+    import torch
+
+    g = G.erdos_renyi(2000, 20000, seed=5)
+    off = torch.from_numpy(g.off.astype(np.int64)).cuda()
+    ind = torch.from_numpy(g.ind.astype(np.int64)).cuda()
+    out = torch.full((g.n,), -7, dtype=torch.int64, device=off.device)
+    assert T.kernel_call64(64, 256, g.n, off, ind, out, 4, 64, 2) == 0
+    torch.cuda.synchronize()
+    ref, _ = oracle.per_vertex(g.off, g.ind)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    assert T.kernel_call64(64, 100, g.n, off, ind, out, 4, 25, 2) == T.TC_EINVAL
+    assert T.kernel_call64(64, 256, g.n, off, ind, out, 4, 64, 7) == T.TC_EINVAL
+
+
 def test_validate(counter):
     import torch
 
