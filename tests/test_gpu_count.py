@@ -344,6 +344,33 @@ def test_int64_csr(counter, cfg):
         assert tot == rtot
 
 
+def test_int64_beyond_2_31_arcs(counter):
+    """An int64 CSR with more than 2^31 arcs, built on the device: disjoint
+    K_64 cliques (every vertex in C(63, 2) triangles), 2^31 + 64512 arcs.
+    Closed form out[v] = 63 * 62 = 3906; the total is 3906 nv."""
+    import torch
+
+    k = 64
+    ncl = 2**31 // (k * (k - 1)) + 16
+    nv, nnz = ncl * k, ncl * k * (k - 1)
+    assert nnz > 2**31
+    dev = torch.device("cuda")
+    off = torch.arange(nv + 1, dtype=torch.int64, device=dev) * (k - 1)
+    j = torch.arange(k - 1, dtype=torch.int64, device=dev)
+    i = torch.arange(k, dtype=torch.int64, device=dev).unsqueeze(1)
+    local = j + (j >= i).to(torch.int64)  # row i of a clique: the other 63 members, ascending
+    ind = (torch.arange(ncl, dtype=torch.int64, device=dev).view(-1, 1, 1) * k + local.unsqueeze(0)).reshape(-1)
+    del local
+    assert ind.numel() == nnz
+    print(f"case int64 cliques nv={nv} nnz={nnz}", flush=True)
+    out, tot = counter.count(off, ind)
+    torch.cuda.synchronize()
+    assert int(tot.item()) == nv * (k - 1) * (k - 2)
+    assert bool((out == (k - 1) * (k - 2)).all().item())
+    del off, ind, out
+    torch.cuda.empty_cache()
+
+
 def test_int64_errors(counter):
     import torch
 
