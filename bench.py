@@ -40,6 +40,9 @@ def make_graph(workload: str):
     if workload == "rmat24":
         return G.rmat(24, 16, seed=1), dict(workload="rmat_s24_ef16", scale=24, edge_factor=16, a=0.57, b=0.19,
                                             c=0.19, seed=1)
+    if workload == "rmat26":
+        return G.rmat(26, 16, seed=1), dict(workload="rmat_s26_ef16", scale=26, edge_factor=16, a=0.57, b=0.19,
+                                            c=0.19, seed=1)
     if workload == "rmat20":
         return G.rmat(20, 16, seed=1), dict(workload="rmat_s20_ef16", scale=20, edge_factor=16, a=0.57, b=0.19,
                                             c=0.19, seed=1)
@@ -325,7 +328,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rgg", "er", "batch", "serve"])
+    ap.add_argument("--workload", default="rmat24", choices=["rmat24", "rmat20", "rmat26", "rgg", "er", "batch", "serve"])
+    ap.add_argument("--int64", action="store_true", help="time the int64 CSR entry point (tc_count_i64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=8,ring=4096")
@@ -406,6 +410,16 @@ def main() -> int:
     log(f"checks: invariants={inv_ok} total={tot} global-mode total={tot_global} ({global_ms:.1f} ms) "
         f"edge-scheme total={tot_edges} per-vertex identical={edges_identical}")
 
+    int64_identical = None
+    if args.int64:  # the same graph as an int64 CSR through tc_count_i64
+        off, ind = off.to(torch.int64), ind.to(torch.int64)
+        out64 = torch.empty(g.n, dtype=torch.int64, device=dev)
+        tc.count(off, ind, out64, total, cfg=cfg)
+        torch.cuda.synchronize()
+        int64_identical = bool(torch.equal(out64, out.to(torch.int64)))
+        out = out64
+        log(f"int64 CSR: per-vertex identical={int64_identical}")
+
     torch.cuda.synchronize()
     kern = []
     prep = []
@@ -451,9 +465,10 @@ def main() -> int:
     line = {
         "metric": METRIC, "value": value, "unit": "edge-intersections/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32",
+        "vs_baseline": None, "dtype": "int64" if args.int64 else "int32",
         "data": "synthetic: seeded R-MAT/ER/RGG from the package's counter-based generators",
         "config": dict(config, n=g.n, arcs=g.nnz, undirected_edges=edges, intersected_edges=E, merge_work=W,
+                       csr="int64" if args.int64 else "int32",
                        kernel_cfg=tc.default_cfg() if cfg is None else cfg),
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -467,6 +482,7 @@ def main() -> int:
         "oriented": {k: ab[k] for k in ("oriented_arcs", "oriented_stream_elements", "max_out_degree")} if oriented
         else None,
         "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
+                   "int64_csr_per_vertex_identical": int64_identical,
                    "global_mode_count_ms": round(global_ms, 3), "edge_scheme_total": tot_edges,
                    "edge_scheme_per_vertex_identical": edges_identical,
                    "host_edge_count_agrees": True if oriented else ab["edges"] == E,
