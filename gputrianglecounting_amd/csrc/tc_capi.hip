@@ -111,7 +111,7 @@ int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz) {
 // edge list (at most nnz/2 edges for a symmetric CSR; in the vertex-list
 // mode every arc of a listed vertex is one, at most nnz); the oriented
 // scheme keeps the orientation in place of the input rows.
-enum class Scheme { kEdges, kList, kOrient };
+enum class Scheme { kEdges, kList, kOrient, kOrientList };
 
 Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
 
@@ -129,7 +129,8 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
     p += align_up(bytes);
     return at;
   };
-  const bool edges = sc != Scheme::kOrient, orient = sc == Scheme::kOrient, list = sc == Scheme::kList;
+  const bool orient = sc == Scheme::kOrient || sc == Scheme::kOrientList, edges = !orient;
+  const bool list = sc == Scheme::kList, counts = list || sc == Scheme::kOrientList;
   L.sc = sc;
   L.ecap = edges ? (list ? nnz + 1 : nnz / 2 + 1) : 0;
   L.voff = take(8 * (size_t)(nv + 1));
@@ -147,7 +148,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.oind = take(orient ? 4 * (size_t)(nnz + 1) : 0);
   L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
-  L.vcnt = take(list ? 4 * (size_t)(nv + 1) : 0);
+  L.vcnt = take(counts ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
   return L;
 }
@@ -192,12 +193,13 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.qctr = reinterpret_cast<unsigned int*>(b + L.qctr);
   p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
   p.ecap = L.ecap;
-  const bool orient = L.sc == Scheme::kOrient, list = L.sc == Scheme::kList;
+  const bool orient = L.sc == Scheme::kOrient || L.sc == Scheme::kOrientList, list = L.sc == Scheme::kList;
+  const bool counts = list || L.sc == Scheme::kOrientList;
   p.orow = orient ? reinterpret_cast<int64_t*>(b + L.orow) : nullptr;
   p.oind = orient ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
   p.prow = orient ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
   p.vmark = list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
-  p.vcnt = list ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
+  p.vcnt = counts ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
 }
 
@@ -480,6 +482,33 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  // A list holding an eighth of the vertices or more (repeats counted) is
+  // cheaper as the whole oriented count gathered per position: the value of
+  // a listed vertex is the same 2 T(v).  Shorter lists intersect only the
+  // listed vertices' arcs.
+  if (cfg.oriented && (int64_t)nlist * 8 >= nv) {
+    const Layout L = layout(nv, nnz, false, Scheme::kOrientList);
+    int rc = ensure(c, L.total);
+    if (rc) return rc;
+    tcd::Prep p = make_prep(c, L, false);
+    record(c, 0, s);
+    if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_fill_i32(p.vcnt, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
+    if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    record(c, 1, s);
+    const int groups = orient_groups(cfg, nv, nnz);
+    if ((e = tcd::launch_count_orient(p, nv, p.vcnt, nullptr, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "count kernel launch");
+    c->last_dscal = p.dscal;
+    record(c, 2, s);
+    if ((e = tcd::launch_gather_list(d_list, nlist, nv, p.vcnt, d_out, d_total, s)) != hipSuccess)
+      return hip_fail(c, e, "gather");
+    record(c, 3, s);
+    c->ev_recorded = c->timing;
+    return TC_OK;
+  }
   const Layout L = layout(nv, nnz, false, Scheme::kList);
   int rc = ensure(c, L.total);
   if (rc) return rc;

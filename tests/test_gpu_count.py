@@ -216,9 +216,13 @@ def _list_expect(g, vl):
     return exp, int(exp[ok].sum())
 
 
-@pytest.mark.parametrize("cfg", [None, dict(search_ratio=1), dict(search_ratio=-1), dict(algo=3), dict(algo=2),
-                                 dict(algo=1), dict(small_max=2048)], ids=str)
+@pytest.mark.parametrize("cfg", [None, dict(row_groups=32), dict(oriented=-1), dict(oriented=-1, search_ratio=1),
+                                 dict(oriented=-1, search_ratio=-1), dict(algo=3), dict(algo=2), dict(algo=1),
+                                 dict(oriented=-1, small_max=2048)], ids=str)
 def test_vertex_list(counter, cfg):
+    """Lists of an eighth of the vertices or more take the oriented count
+    and a gather, shorter ones (and oriented = -1) the listed vertices'
+    arcs; both give the oracle's vertex-list values."""
     import torch
 
     g = G.rmat(13, 32, seed=3)
