@@ -14,10 +14,12 @@
 // gpurun.cu:49-51,
 //   nv <tab> total <tab> memAllocMs <tab> tCountMs <tab> execMs
 // where total is the int64 sum of the per-vertex counts (6·T).  The legacy
-// launch shape is validated exactly as param.cu:225-247 does; the tuned
-// two-level kernel runs.  Without a block size it sweeps the compiled
-// kernel variants instead of the reference's 48 legacy shapes and writes
-// "<graph>.o.<blocks>" rows: wg_size items ring allocMs execMs sumMs total.
+// launch shape is validated exactly as param.cu:225-247 does; the default
+// count (degree-oriented) runs.  Without a block size it sweeps the
+// oriented scheme's partners per wave and the compiled edge-scheme kernel
+// variants instead of the reference's 48 legacy shapes and writes
+// "<graph>.o.<blocks>" rows: scheme algo wg_size items ring chunk depth
+// row_groups allocMs execMs sumMs total.
 // Inputs that are not simple undirected graphs are normalised (self-loops
 // and duplicate arcs dropped, reverse arcs added) with a warning.
 #include <hip/hip_runtime.h>
@@ -236,26 +238,34 @@ int main(int argc, char** argv) {
     HIPCHECK(hipMemcpy(&total, d_total, 8, hipMemcpyDeviceToHost));
     std::printf("%d\t%lld\t%g\t%g\t%g\n", nv, (long long)total, alloc_ms, count_ms, ms_since(t0));
   } else {
+    // next to the graph file, as gpurun.cu:146-149; a generator spec names a file in the working directory
     std::string base = spec;
-    for (char& ch : base)
-      if (ch == ':' || ch == '/') ch = '_';
+    if (spec.rfind("gen:", 0) == 0)
+      for (char& ch : base)
+        if (ch == ':' || ch == '/') ch = '_';
     std::string fname = base + ".o." + std::to_string(blocks);
     FILE* f = std::fopen(fname.c_str(), "w");
     if (!f) {
       std::fprintf(stderr, "cannot write %s\n", fname.c_str());
       return 1;
     }
-    static const int kVar[][6] = {{0, 256, 8, 4096, 16, 2}, {0, 256, 8, 4096, 16, 4}, {0, 128, 16, 4096, 16, 2},
-                                  {0, 512, 4, 4096, 16, 2},  {3, 256, 8, 4096, 16, 2}, {3, 256, 8, 4096, 16, 4},
-                                  {3, 512, 4, 4096, 16, 2},  {3, 128, 16, 4096, 16, 2}, {2, 256, 8, 4096, 0, 0}};
+    // {oriented, algo, wg_size, items, ring, chunk, depth, row_groups}; 0 = the context default
+    static const int kVar[][8] = {
+        {1, 0, 0, 0, 0, 0, 0, 2},          {1, 0, 0, 0, 0, 0, 0, 4},          {1, 0, 0, 0, 0, 0, 0, 8},
+        {1, 0, 0, 0, 0, 0, 0, 16},         {1, 0, 0, 0, 0, 0, 0, 32},         {-1, 0, 256, 8, 4096, 16, 2, 0},
+        {-1, 0, 256, 8, 4096, 16, 4, 0},   {-1, 0, 128, 16, 4096, 16, 2, 0},  {-1, 0, 512, 4, 4096, 16, 2, 0},
+        {-1, 3, 256, 8, 4096, 16, 2, 0},   {-1, 3, 256, 8, 4096, 16, 4, 0},   {-1, 3, 512, 4, 4096, 16, 2, 0},
+        {-1, 3, 128, 16, 4096, 16, 2, 0},  {-1, 2, 256, 8, 4096, 0, 0, 0}};
     for (const auto& v : kVar) {
       tc_cfg cfg{};
-      cfg.algo = v[0];
-      cfg.wg_size = v[1];
-      cfg.items = v[2];
-      cfg.ring = v[3];
-      cfg.chunk = v[4];
-      cfg.depth = v[5];
+      cfg.oriented = v[0];
+      cfg.algo = v[1];
+      cfg.wg_size = v[2];
+      cfg.items = v[3];
+      cfg.ring = v[4];
+      cfg.chunk = v[5];
+      cfg.depth = v[6];
+      cfg.row_groups = v[7];
       auto t1 = clk::now();
       rc = tc_count_i32(ctx, nv, nnz, d_off, d_ind, d_out, d_total, &cfg, nullptr);
       int64_t total = 0;
@@ -265,9 +275,8 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "count failed: %s\n", tc_last_error(ctx));
         break;
       }
-      std::fprintf(f, "%d\t%d\t%d\t%d\t%d\t%d\t%g\t%g\t%g\t%lld\n", v[0], v[1], v[2], v[3], v[4], v[5], alloc_ms, ex,
-                   ex + alloc_ms,
-                   (long long)total);
+      std::fprintf(f, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%g\t%g\t%g\t%lld\n", v[0] > 0 ? "oriented" : "edges", v[1], v[2],
+                   v[3], v[4], v[5], v[6], v[7], alloc_ms, ex, ex + alloc_ms, (long long)total);
     }
     std::fclose(f);
     std::printf("wrote %s\n", fname.c_str());

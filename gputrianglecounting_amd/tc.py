@@ -229,7 +229,7 @@ class TriangleCounter:
         return Timing(*[float(x) for x in ms])
 
     def last_edge_work(self):
-        """(E, W) of the last count: edges intersected and their total Merge Path length."""
+        """(E, W) of the last edge-scheme count: edges intersected and their total merge length ((0, 0) after an oriented count)."""
         buf = (C.c_int64 * 2)()
         self._check(lib().tc_last_edge_work(self._h, buf), "edge work")
         return int(buf[0]), int(buf[1])

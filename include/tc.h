@@ -34,8 +34,10 @@ extern "C" {
 
 typedef struct tc_ctx tc_ctx;
 
-/* Launch configuration of the two-level Merge Path kernel.  Zero fields
- * take the tuned default.
+/* Count configuration.  Zero fields take the context's default.  The
+ * default counts on the degree-oriented scheme (`oriented`, `row_groups`);
+ * the other fields size the edge-list scheme's kernels (oriented = -1, or
+ * algo != 0).
  *   wg_size        threads per workgroup (256 or 512)
  *   items          merge-path steps per lane per tile (8, 16 or 32)
  *   ring           LDS ring length per adjacency list, elements (power of 2);
@@ -117,8 +119,8 @@ int tc_ctx_set_default_cfg(tc_ctx* ctx, const tc_cfg* cfg);
 int tc_ctx_get_default_cfg(const tc_ctx* ctx, tc_cfg* cfg);
 /* When enabled, each count records HIP events around its phases on the
  * launch stream; tc_last_timing returns (after the stream is synchronised)
- * ms[0] = edge-list prep, ms[1] = the Merge Path count kernel,
- * ms[2] = finalisation, ms[3] = whole call. */
+ * ms[0] = prep (the orientation, or the edge list), ms[1] = the count
+ * kernels, ms[2] = finalisation, ms[3] = whole call. */
 int tc_ctx_set_timing(tc_ctx* ctx, int enable);
 int tc_last_timing(tc_ctx* ctx, float ms[4]);
 
@@ -137,9 +139,10 @@ int tc_count_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, con
  * intersection) one of {1,2,4,8,16,32} with number_blocks == numberThreads /
  * threads_per_block and shifter == log2(threads_per_block), as
  * param.cu:225-247 requires; otherwise TC_EINVAL (the reference silently
- * launches nothing).  The legacy launch shape is validated, then the tuned
- * two-level kernel runs on the default stream of the current device with a
- * library-owned context.  outPutTriangles is fully written. */
+ * launches nothing).  The legacy launch shape is validated, then the
+ * default count (the degree-oriented scheme) runs on the default stream of
+ * the current device with a library-owned context.  outPutTriangles is
+ * fully written. */
 int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv, const int32_t* d_off,
                       const int32_t* d_ind, int32_t* outPutTriangles, int32_t threads_per_block,
                       int32_t number_blocks, int32_t shifter);
@@ -254,9 +257,10 @@ int tc_autotune_batch_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t 
 int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
                        int64_t* d_total, void* stream);
 
-/* Edge count E and total Merge Path work W = sum over intersected edges of
- * |N(u)| + |N(w)| of the last count on this context (synchronises the
- * context's device).  out[0] = E, out[1] = W. */
+/* Edge count E and total merge work W = sum over intersected edges of
+ * |N(u)| + |N(w)| of the last edge-list-scheme count on this context
+ * (synchronises the context's device); (0, 0) after an oriented count,
+ * which builds no edge list.  out[0] = E, out[1] = W. */
 int tc_last_edge_work(tc_ctx* ctx, int64_t out[2]);
 
 /* Plain device copy d_dst[i] = d_src[i] of n int32 values (one read and one

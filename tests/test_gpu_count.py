@@ -471,3 +471,23 @@ def test_tc_main_partition_run(tmp_path):
     assert int(nv) == g.n and int(total) == rtot
     r = subprocess.run([exe, base, "1", "8", "256", "4"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and int(r.stdout.split()[1]) == rtot
+
+
+def test_tc_main_sweep(tmp_path):
+    """tc_main <graph> 1 <blocks> (no block size): the sweep file
+    <graph>.o.<blocks> has one row per oriented partners-per-wave setting and
+    per edge-scheme variant, every total the oracle's."""
+    import os
+    import subprocess
+
+    g = G.rmat(11, 16, seed=9)
+    base = str(tmp_path / "g")
+    g.write_info(base + ".info")
+    exe = os.path.join(os.path.dirname(T.__file__), "_lib", "tc_main")
+    print("case tc_main sweep", flush=True)
+    r = subprocess.run([exe, base, "1", "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split("\t") for line in open(base + ".o.8").read().splitlines()]  # next to the graph
+    _, rtot = _ref(g)
+    assert {row[0] for row in rows} == {"oriented", "edges"} and len(rows) == 14
+    assert all(int(row[-1]) == rtot for row in rows)
