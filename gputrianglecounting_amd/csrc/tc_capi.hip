@@ -95,16 +95,20 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 }
 
 // Partners a wave streams at once on the oriented scheme when row_groups
-// asks for the automatic choice (<= 0).  Sparser and smaller graphs have
-// shorter N+ lists, which want more groups of fewer lanes; measured on
-// R-MAT s24 (4 best), R-MAT s20 (8), the mixed 4096-graph batch (16) and
-// RGG n=4M (32).
-int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz) {
+// asks for the automatic choice (<= 0).  Larger power-law graphs have
+// longer N+ lists, which want fewer groups of more lanes; sparser graphs
+// want more.  Measured best: R-MAT s26 1 (1335 ms; 2: 1562), s24 2 (209 ms;
+// 4: 222), s20 8, the mixed 4096-graph batch 16, RGG n=4M 16-32.  A batch
+// is sized by its mean degree only: its lists are those of its (small)
+// graphs, whatever the batch's total.
+int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false) {
   if (cfg.row_groups > 0) return cfg.row_groups;
+  if (!batch && nnz >= (int64_t(1) << 30)) return 1;
+  if (!batch && nnz >= (int64_t(1) << 28)) return 2;
   const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
   if (dbar < 10) return 32;
   if (dbar < 20) return 16;
-  return nnz >= (int64_t(1) << 28) ? 4 : 8;
+  return 8;
 }
 
 // Scratch layout of one call.  The edge-list scheme keeps an undirected
@@ -562,7 +566,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
-    const int groups = orient_groups(cfg, nv_total, nnz_total);
+    const int groups = orient_groups(cfg, nv_total, nnz_total, true);
     if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s)) !=
         hipSuccess)
       return hip_fail(c, e, "count kernel launch");
@@ -741,7 +745,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
   const Scheme sc = scheme_of(cfg);
-  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz), 1, c->cus);
+  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz, batch), 1, c->cus);
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 

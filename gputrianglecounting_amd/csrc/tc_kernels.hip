@@ -928,7 +928,7 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
 // oriented count kernel: the window sets the LDS per workgroup (24 (AW + 1)
 // bytes), which with the registers bounds the resident waves.
 constexpr int kOrientAW = 851;
-constexpr bool kOrientPF = false;
+constexpr bool kOrientPF = true;
 constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,

@@ -43,11 +43,12 @@
 //
 // Occupancy decides this kernel's speed (it waits on memory and LDS, and
 // the time falls in proportion to the waves per SIMD), so the lane's merge
-// is kept small: 32-bit chunk and element offsets from a 64-byte aligned
-// base, one chunk in registers (PF: plus the next one in flight).
+// is kept small: 32-bit chunk and element offsets from a chunk-aligned
+// base, and chunks of 8 elements (32 bytes), the current one in registers
+// and (PF) the next one in flight.
 
 // Lane part of one list of the orientation: chunks [c0, c1) of CH elements
-// from bp (64-byte aligned); elements before offset e0 or from e1 on are
+// from bp (CH * 4-byte aligned); elements before offset e0 or from e1 on are
 // masked out.  A: the staged window (n elements, A[n] = INT_MAX).  Every
 // match at A position p adds 2 to X's 16-bit counter p; returns the
 // matches.
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
     unsigned long long* ctr, int by_in) {
   constexpr int NW = 4;
-  constexpr int CH = 16;
+  constexpr int CH = 8;
   constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
   static_assert(2 * P < 65536, "a piece's counts must fit 16 bits");
   __shared__ int32_t abuf_all[NW * (AW + 1)];

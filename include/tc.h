@@ -77,9 +77,9 @@ typedef struct tc_ctx tc_ctx;
  *                  which builds no edge list.
  *   row_groups     oriented scheme: partners of a row a wave streams at once,
  *                  each by 64/row_groups lanes (1, 2, 4, 8, 16 or 32;
- *                  default -1 = chosen from the mean degree and size:
- *                  32 below mean degree 10, 16 below 20, else 4 from 2^28
- *                  arcs up and 8 below) */
+ *                  default -1 = chosen from the size and mean degree: 1
+ *                  from 2^30 arcs up, 2 from 2^28 (single graphs), else 32
+ *                  below mean degree 10, 16 below 20, 8 above) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
