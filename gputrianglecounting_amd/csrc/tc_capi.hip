@@ -567,8 +567,8 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv_total, nnz_total, true);
-    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s)) !=
-        hipSuccess)
+    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s,
+                                      tcd::kOrientPieceBatch)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)

@@ -39,8 +39,13 @@ struct Variant {
 int num_variants();
 Variant variant(int i);
 
-// Partner arcs per work piece of the oriented count kernel.
+// Arc positions per work piece of the oriented count kernel: the smallest
+// used (single graphs) and the batches' (their many short rows and small
+// graphs run faster in longer pieces).  A piece's match counts must fit 16
+// bits: 2 * piece < 65536.
 constexpr int64_t kOrientPiece = 256;
+constexpr int64_t kOrientPieceBatch = 1024;
+static_assert(2 * kOrientPieceBatch < 65536 && kOrientPiece <= kOrientPieceBatch, "piece sizes");
 
 // Scratch sizing and prep/count launchers (stream-ordered, no host sync).
 struct Prep {
@@ -101,9 +106,9 @@ hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int3
 // arc (partners are its in-neighbours) instead of the lower one (partners
 // are its out-neighbours).
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s);
+                               int cus, hipStream_t s, int64_t piece = kOrientPiece);
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s);
+                               int cus, hipStream_t s, int64_t piece = kOrientPiece);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

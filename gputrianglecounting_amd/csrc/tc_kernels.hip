@@ -932,18 +932,17 @@ constexpr bool kOrientPF = true;
 constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
-                          unsigned long long*, unsigned long long*, int);
+                          unsigned long long*, unsigned long long*, int, int32_t);
 // partners per wave (groups), per-vertex output type (or the total only) -> kernel
 template <bool PV, typename OT>
 OrientFn<OT> orient_fn(int groups) {
-  constexpr int P = (int)kOrientPiece;
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, P, kOrientPF, kOrientWPE, PV, OT>;
-    case 2: return k_count_orient<kOrientAW, 2, P, kOrientPF, kOrientWPE, PV, OT>;
-    case 8: return k_count_orient<kOrientAW, 8, P, kOrientPF, kOrientWPE, PV, OT>;
-    case 16: return k_count_orient<kOrientAW, 16, P, kOrientPF, kOrientWPE, PV, OT>;
-    case 32: return k_count_orient<kOrientAW, 32, P, kOrientPF, kOrientWPE, PV, OT>;
-    default: return k_count_orient<kOrientAW, 4, P, kOrientPF, kOrientWPE, PV, OT>;
+    case 1: return k_count_orient<kOrientAW, 1, kOrientPF, kOrientWPE, PV, OT>;
+    case 2: return k_count_orient<kOrientAW, 2, kOrientPF, kOrientWPE, PV, OT>;
+    case 8: return k_count_orient<kOrientAW, 8, kOrientPF, kOrientWPE, PV, OT>;
+    case 16: return k_count_orient<kOrientAW, 16, kOrientPF, kOrientWPE, PV, OT>;
+    case 32: return k_count_orient<kOrientAW, 32, kOrientPF, kOrientWPE, PV, OT>;
+    default: return k_count_orient<kOrientAW, 4, kOrientPF, kOrientWPE, PV, OT>;
   }
 }
 // kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts
@@ -968,29 +967,30 @@ int resident_orient_wgs(int groups, int kind, int cus) {
 
 template <bool PV, typename OT>
 hipError_t launch_count_orient_t(const Prep& p, int64_t nv, OT* out, int64_t* total, int groups, bool by_in, int kind,
-                                 int cus, hipStream_t s) {
+                                 int cus, hipStream_t s, int64_t piece) {
+  if (piece < kOrientPiece || 2 * piece >= 65536) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
-                     kOrientPiece, p.prow);
+  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, piece,
+                     p.prow);
   const OrientFn<OT> fn = orient_fn<PV, OT>(groups);
   hipLaunchKernelGGL(fn, dim3(resident_orient_wgs(groups, kind, cus)), dim3(256), 0, s,
                      (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.prow, nv,
-                     out, reinterpret_cast<unsigned long long*>(total), ctr, by_in ? 1 : 0);
+                     out, reinterpret_cast<unsigned long long*>(total), ctr, by_in ? 1 : 0, (int32_t)piece);
   return hipGetLastError();
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s) {
-  return out ? launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 1, cus, s)
-             : launch_count_orient_t<false>(p, nv, out, total, groups, by_in, 0, cus, s);
+                               int cus, hipStream_t s, int64_t piece) {
+  return out ? launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 1, cus, s, piece)
+             : launch_count_orient_t<false>(p, nv, out, total, groups, by_in, 0, cus, s, piece);
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s) {
-  if (!out) return launch_count_orient_t<false>(p, nv, (int32_t*)nullptr, total, groups, by_in, 0, cus, s);
-  return launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 2, cus, s);
+                               int cus, hipStream_t s, int64_t piece) {
+  if (!out) return launch_count_orient_t<false>(p, nv, (int32_t*)nullptr, total, groups, by_in, 0, cus, s, piece);
+  return launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 2, cus, s, piece);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {

@@ -116,15 +116,14 @@ __device__ __forceinline__ void out_add(int64_t* out, int64_t i, int64_t v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(out + i), (unsigned long long)v);
 }
 
-template <int AW, int G, int P, bool PF, int WPE, bool PV, typename OT>
+template <int AW, int G, bool PF, int WPE, bool PV, typename OT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
-    unsigned long long* ctr, int by_in) {
+    unsigned long long* ctr, int by_in, int32_t P) {
   constexpr int NW = 4;
   constexpr int CH = 8;
   constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
-  static_assert(2 * P < 65536, "a piece's counts must fit 16 bits");
   __shared__ int32_t abuf_all[NW * (AW + 1)];
   __shared__ int32_t xcnt_all[PV ? NW * XW : 1];
   const int lane = threadIdx.x & 63;
