@@ -406,11 +406,11 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   return TC_OK;
 }
 
-int tc_count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind, int64_t* d_out,
-                 int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
-  if (!c) return TC_EINVAL;
-  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || (nv > 0 && !d_out))
-    return set_err(c, TC_EINVAL, "invalid argument");
+namespace {
+// The int64 CSR on the oriented scheme: per-vertex int64 counts into d_out,
+// or the total only when d_out is null.
+int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind, int64_t* d_out,
+              int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
   if (nv >= INT32_MAX) return set_err(c, TC_ERANGE, "more than 2^31 - 1 vertices");
   tc_cfg cfg;
   if (resolve(c, cfg_in, &cfg) < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
@@ -424,7 +424,7 @@ int tc_count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const
   tcd::Prep p = make_prep(c, L, false);
   p.voff = const_cast<int64_t*>(d_off);  // the caller's offsets are already int64; the kernels only read them
   record(c, 0, s);
-  if ((e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if (d_out && (e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
@@ -437,6 +437,23 @@ int tc_count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const
   record(c, 3, s);
   c->ev_recorded = c->timing;
   return TC_OK;
+}
+
+}  // namespace
+
+int tc_count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind, int64_t* d_out,
+                 int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || (nv > 0 && !d_out))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  return count_i64(c, nv, nnz, d_off, d_ind, d_out, d_total, cfg_in, stream);
+}
+
+int tc_count_total_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind,
+                       int64_t* d_total, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || !d_total) return set_err(c, TC_EINVAL, "invalid argument");
+  return count_i64(c, nv, nnz, d_off, d_ind, nullptr, d_total, nullptr, stream);
 }
 
 int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,

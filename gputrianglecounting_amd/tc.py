@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
         "tc_count_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_kernelCall_i32": (C.c_int, [C.c_uint, C.c_uint, i32, vp, vp, vp, i32, i32, i32]),
         "tc_count_i64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_count_total_i64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp]),
         "tc_kernelCall_i64": (C.c_int, [C.c_uint, C.c_uint, i64, vp, vp, vp, i64, i64, i64]),
         "tc_lcc_f64": (C.c_int, [vp, i32, vp, vp, vp, vp]),
         "tc_validate_i32": (C.c_int, [vp, i32, i64, vp, vp, C.POINTER(C.c_int), vp]),
@@ -134,7 +135,8 @@ def lib() -> C.CDLL:
 # Every symbol include/tc.h declares (checked by the CPU test suite).
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
-           "tc_count_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_lcc_f64", "tc_validate_i32",
+           "tc_count_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_count_total_i64", "tc_lcc_f64",
+           "tc_validate_i32",
            "tc_batch_count_i32",
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
@@ -284,6 +286,11 @@ class TriangleCounter:
         nnz = int(ind.shape[0])
         if total is None:
             total = torch.empty(1, dtype=torch.int64, device=off.device)
+        if off.dtype == torch.int64 and ind.dtype == torch.int64:
+            rc = lib().tc_count_total_i64(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(total),
+                                          _stream(stream))
+            self._check(rc, "tc_count_total_i64")
+            return total
         rc = lib().tc_count_total_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(total),
                                       _stream(stream))
         self._check(rc, "tc_count_total_i32")

@@ -256,6 +256,9 @@ int tc_autotune_batch_i32(tc_ctx* ctx, int32_t G, const int64_t* d_tab, int64_t 
  * degree-ordered orientation so each triangle is intersected once. */
 int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
                        int64_t* d_total, void* stream);
+/* The same for an int64 CSR (as tc_count_i64: any nnz, ids below 2^31). */
+int tc_count_total_i64(tc_ctx* ctx, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind,
+                       int64_t* d_total, void* stream);
 
 /* Edge count E and total merge work W = sum over intersected edges of
  * |N(u)| + |N(w)| of the last edge-list-scheme count on this context

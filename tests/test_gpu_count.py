@@ -346,6 +346,12 @@ def test_int64_csr(counter, cfg):
         ref, rtot = _ref(g)
         np.testing.assert_array_equal(out, ref)
         assert tot == rtot
+        if cfg is None:
+            import torch
+
+            off = torch.from_numpy(g.off.astype(np.int64)).cuda()
+            ind = torch.from_numpy(g.ind.astype(np.int64)).cuda()
+            assert int(counter.count_total(off, ind).item()) == rtot
 
 
 def test_int64_beyond_2_31_arcs(counter):
@@ -371,7 +377,9 @@ def test_int64_beyond_2_31_arcs(counter):
     torch.cuda.synchronize()
     assert int(tot.item()) == nv * (k - 1) * (k - 2)
     assert bool((out == (k - 1) * (k - 2)).all().item())
-    del off, ind, out
+    del out
+    assert int(counter.count_total(off, ind).item()) == nv * (k - 1) * (k - 2)  # tc_count_total_i64
+    del off, ind
     torch.cuda.empty_cache()
 
 
