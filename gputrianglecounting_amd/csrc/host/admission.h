@@ -14,6 +14,7 @@
 #include <deque>
 #include <iterator>
 #include <map>
+#include <thread>
 #include <vector>
 
 namespace tcs {
@@ -106,9 +107,12 @@ inline BatchLayout batch_layout(int64_t G, int64_t nv, int64_t nnz) {
 
 // Writes graphs 0..G-1 (nv[g] vertices, nnz[g] arcs, offsets off[g][0..nv],
 // ids ind[g][0..nnz)) into the [0, upload_end) part of a host block laid
-// out by batch_layout(G, sum nv, sum nnz); returns that layout.
+// out by batch_layout(G, sum nv, sum nnz); returns that layout.  The table
+// is written first; the copies, which dominate, are then split by bytes
+// over up to `threads` threads (one when the block is small) — every byte
+// lands where the serial copy would put it.
 inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
-                              const int32_t* const* ind, char* blk) {
+                              const int32_t* const* ind, char* blk, int threads = 8) {
   int64_t v = 0, a = 0;
   for (int32_t g = 0; g < G; ++g) {
     v += nv[g];
@@ -122,13 +126,48 @@ inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, 
   for (int32_t g = 0; g < G; ++g) {
     tab[2 * g] = v;
     tab[2 * g + 1] = a;
-    std::memcpy(offc + v + g, off[g], 4 * ((size_t)nv[g] + 1));
-    if (nnz[g]) std::memcpy(indc + a, ind[g], 4 * (size_t)nnz[g]);
     v += nv[g];
     a += nnz[g];
   }
   tab[2 * G] = v;
   tab[2 * G + 1] = a;
+  // copy tasks: graph g's offsets and ids, each cut into pieces of at most `cut` bytes
+  struct Task {
+    char* dst;
+    const char* src;
+    size_t n;
+  };
+  std::vector<Task> tasks;
+  const size_t total = 4 * (size_t)(v + G + a);
+  const int nt = total < (size_t(16) << 20) || threads < 2 ? 1 : threads;
+  const size_t cut = nt > 1 ? (total + (size_t)nt - 1) / (size_t)nt : total + 1;
+  auto add = [&](void* dst, const void* src, size_t n) {
+    for (size_t o = 0; o < n; o += cut)
+      tasks.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, n - o < cut ? n - o : cut});
+  };
+  for (int32_t g = 0; g < G; ++g) {
+    add(offc + tab[2 * g] + g, off[g], 4 * ((size_t)nv[g] + 1));
+    if (nnz[g]) add(indc + tab[2 * g + 1], ind[g], 4 * (size_t)nnz[g]);
+  }
+  if (nt == 1) {
+    for (const Task& t : tasks) std::memcpy(t.dst, t.src, t.n);
+    return L;
+  }
+  // thread k copies the tasks whose bytes fall in its share, in order
+  std::vector<size_t> start(tasks.size() + 1, 0);
+  for (size_t i = 0; i < tasks.size(); ++i) start[i + 1] = start[i] + tasks[i].n;
+  std::vector<std::thread> pool;
+  size_t i0 = 0;
+  for (int k = 0; k < nt && i0 < tasks.size(); ++k) {
+    const size_t hi = (total * (size_t)(k + 1)) / (size_t)nt;
+    size_t i1 = i0;
+    while (i1 < tasks.size() && (start[i1] < hi || k == nt - 1)) ++i1;
+    pool.emplace_back([&tasks, i0, i1] {
+      for (size_t i = i0; i < i1; ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
+    });
+    i0 = i1;
+  }
+  for (auto& t : pool) t.join();
   return L;
 }
 

@@ -219,3 +219,24 @@ def test_server_block_matches_batch_layout():
     # the last row of each graph
     for g in range(Gn):
         assert b.tab[2 * g + 1] + graphs[g].nnz == b.tab[2 * g + 3]
+
+
+def test_server_block_parallel_copy():
+    """A block of more than 16 MiB is packed by several threads; every byte
+    matches tch_pack's batch."""
+    from gputrianglecounting_amd import graph as G
+
+    graphs = [G.rmat(17, 16, seed=4), G.from_csr([0, 0], []), G.erdos_renyi(200000, 1500000, seed=5),
+              G.rmat(12, 8, seed=6)]
+    blk, n, size = _pack_block(graphs)
+    b = G.pack(graphs)
+    assert 4 * (b.total_arcs + b.total_vertices) > (16 << 20)
+    Gn = len(graphs)
+    tab_bytes = 16 * (Gn + 1)
+    off_at = -(-tab_bytes // 256) * 256
+    off_bytes = 4 * (b.total_vertices + Gn)
+    ind_at = off_at + -(-off_bytes // 256) * 256
+    assert n == ind_at + 4 * b.total_arcs and n <= size
+    np.testing.assert_array_equal(blk[:tab_bytes].view(np.int64), b.tab)
+    np.testing.assert_array_equal(blk[off_at:off_at + off_bytes].view(np.int32), b.off)
+    np.testing.assert_array_equal(blk[ind_at:n].view(np.int32), b.ind)
