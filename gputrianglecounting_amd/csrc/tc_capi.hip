@@ -150,7 +150,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.scan = take(edges ? 8 * tcd::scan_tmp_elems(std::max<int64_t>(nv + 1, L.ecap)) : 0);
   L.orow = take(orient ? 16 * (size_t)(nv + 1) : 0);
   L.oind = take(orient ? 4 * (size_t)(nnz + 1) : 0);
-  L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2) : 0);
+  L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2 + tcd::kOrientPieceSlack) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(counts ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
@@ -390,7 +390,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
+    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
@@ -430,7 +430,7 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
+  if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -480,8 +480,8 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, static_cast<int32_t*>(nullptr), d_total, groups, false, c->cus, s)) !=
-      hipSuccess)
+  if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups, false, c->cus,
+                                    s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -520,7 +520,8 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, p.vcnt, nullptr, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
+    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, cfg.oriented == 2, c->cus, s)) !=
+        hipSuccess)
       return hip_fail(c, e, "count kernel launch");
     c->last_dscal = p.dscal;
     record(c, 2, s);
@@ -584,7 +585,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv_total, nnz_total, true);
-    if ((e = tcd::launch_count_orient(p, nv_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s,
+    if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s,
                                       tcd::kOrientPieceBatch)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {

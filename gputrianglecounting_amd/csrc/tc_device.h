@@ -39,13 +39,29 @@ struct Variant {
 int num_variants();
 Variant variant(int i);
 
-// Arc positions per work piece of the oriented count kernel: the smallest
-// used (single graphs) and the batches' (their many short rows and small
-// graphs run faster in longer pieces).  A piece's match counts must fit 16
-// bits: 2 * piece < 65536.
+// Arc positions per work piece of the oriented count kernel: single graphs
+// and batches (their many short rows and small graphs run faster in longer
+// pieces).  A graph whose pieces at that size would fit the grid's waves
+// takes the smallest size (down to kOrientPieceMin) that still gives at
+// most one piece per wave, and every wave then counts one piece; prow
+// holds kOrientPieceSlack entries beyond nnz / kOrientPiece for that case.
+// A piece's match counts must fit 16 bits: 2 * piece < 65536.
 constexpr int64_t kOrientPiece = 256;
 constexpr int64_t kOrientPieceBatch = 1024;
-static_assert(2 * kOrientPieceBatch < 65536 && kOrientPiece <= kOrientPieceBatch, "piece sizes");
+constexpr int64_t kOrientPieceMin = 16;
+constexpr int64_t kOrientPieceSlack = 65536;
+static_assert(2 * kOrientPieceBatch < 65536 && kOrientPieceMin <= kOrientPiece && kOrientPiece <= kOrientPieceBatch,
+              "piece sizes");
+// Piece size for nnz arc positions from base (kOrientPiece or
+// kOrientPieceBatch) and the grid's waves; a result below base leaves at
+// most min(waves, kOrientPieceSlack) pieces.
+inline int64_t orient_piece(int64_t nnz, int64_t base, int64_t waves) {
+  if (waves > kOrientPieceSlack) waves = kOrientPieceSlack;
+  if (nnz > waves * base) return base;
+  int64_t p = base;
+  while (p > kOrientPieceMin && (nnz + p / 2 - 1) / (p / 2) <= waves) p /= 2;
+  return p;
+}
 
 // Scratch sizing and prep/count launchers (stream-ordered, no host sync).
 struct Prep {
@@ -68,8 +84,8 @@ struct Prep {
   // then the in-neighbours; orow[2v] = voff[v], orow[2v+1] = end of N+(v).
   int64_t* orow;    // [2 nv] or nullptr
   int32_t* oind;    // [nnz]
-  // row of the first arc of every piece of kOrientPiece arc positions
-  int32_t* prow;    // [nnz / kOrientPiece + 2]
+  // row of the first arc of every piece (orient_piece arc positions)
+  int32_t* prow;    // [nnz / kOrientPiece + 2 + kOrientPieceSlack]
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -101,14 +117,15 @@ int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
 // Per-vertex counts (out; nullptr: the total only) over the orientation in
-// p.orow / p.oind (k_count_orient).  groups: arcs of a row streamed at a
-// time per wave (1, 2, 4 or 8).  by_in: rows are the higher vertex of each
-// arc (partners are its in-neighbours) instead of the lower one (partners
-// are its out-neighbours).
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s, int64_t piece = kOrientPiece);
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s, int64_t piece = kOrientPiece);
+// p.orow / p.oind (k_count_orient) of nnz arcs.  groups: arcs of a row
+// streamed at a time per wave (1, 2, 4, 8, 16 or 32).  by_in: rows are the
+// higher vertex of each arc (partners are its in-neighbours) instead of the
+// lower one (partners are its out-neighbours).  piece: the base piece size
+// (orient_piece picks the one used).
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
+                               bool by_in, int cus, hipStream_t s, int64_t piece = kOrientPiece);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
+                               bool by_in, int cus, hipStream_t s, int64_t piece = kOrientPiece);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

@@ -317,7 +317,9 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
 // scan, and a row never spills into another whatever the input (a CSR that
 // is not symmetric only orients differently).
 //
-// Waves take 64 vertices at a time from ctr.  A row of at most kShortRow
+// Waves take 64 vertices at a time from ctr (the grid has at most one wave
+// per 64 vertices, so a small graph does not crowd the counter with idle
+// waves).  A row of at most kShortRow
 // arcs is done by its own lane, four arcs per step (sparse graphs have
 // rows of a few arcs, which would leave most of a wave idle); the longer
 // rows of the 64 are then done one at a time by the whole wave, 256 arcs
@@ -933,16 +935,17 @@ constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
                           unsigned long long*, unsigned long long*, int, int32_t);
-// partners per wave (groups), per-vertex output type (or the total only) -> kernel
-template <bool PV, typename OT>
+// partners per wave (groups), per-vertex output type (or the total only),
+// one piece per wave -> kernel
+template <bool PV, typename OT, bool ONE = false>
 OrientFn<OT> orient_fn(int groups) {
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, kOrientPF, kOrientWPE, PV, OT>;
-    case 2: return k_count_orient<kOrientAW, 2, kOrientPF, kOrientWPE, PV, OT>;
-    case 8: return k_count_orient<kOrientAW, 8, kOrientPF, kOrientWPE, PV, OT>;
-    case 16: return k_count_orient<kOrientAW, 16, kOrientPF, kOrientWPE, PV, OT>;
-    case 32: return k_count_orient<kOrientAW, 32, kOrientPF, kOrientWPE, PV, OT>;
-    default: return k_count_orient<kOrientAW, 4, kOrientPF, kOrientWPE, PV, OT>;
+    case 1: return k_count_orient<kOrientAW, 1, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 2: return k_count_orient<kOrientAW, 2, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 8: return k_count_orient<kOrientAW, 8, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 16: return k_count_orient<kOrientAW, 16, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 32: return k_count_orient<kOrientAW, 32, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    default: return k_count_orient<kOrientAW, 4, kOrientPF, kOrientWPE, PV, OT, ONE>;
   }
 }
 // kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts
@@ -966,31 +969,34 @@ int resident_orient_wgs(int groups, int kind, int cus) {
 }
 
 template <bool PV, typename OT>
-hipError_t launch_count_orient_t(const Prep& p, int64_t nv, OT* out, int64_t* total, int groups, bool by_in, int kind,
-                                 int cus, hipStream_t s, int64_t piece) {
-  if (piece < kOrientPiece || 2 * piece >= 65536) return hipErrorInvalidValue;
+hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total, int groups,
+                                 bool by_in, int kind, int cus, hipStream_t s, int64_t base) {
+  if (base < kOrientPieceMin || 2 * base >= 65536 || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
+  const int nwg = resident_orient_wgs(groups, kind, cus);
+  const int64_t piece = orient_piece(nnz, base, 4 * (int64_t)nwg);
   hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, piece,
                      p.prow);
-  const OrientFn<OT> fn = orient_fn<PV, OT>(groups);
-  hipLaunchKernelGGL(fn, dim3(resident_orient_wgs(groups, kind, cus)), dim3(256), 0, s,
-                     (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.prow, nv,
-                     out, reinterpret_cast<unsigned long long*>(total), ctr, by_in ? 1 : 0, (int32_t)piece);
+  // a wave for every piece: the kernel that takes one piece per wave
+  const OrientFn<OT> fn = 4 * (int64_t)nwg * piece >= nnz ? orient_fn<PV, OT, true>(groups) : orient_fn<PV, OT>(groups);
+  hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
+                     (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total),
+                     ctr, by_in ? 1 : 0, (int32_t)piece);
   return hipGetLastError();
 }
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int32_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s, int64_t piece) {
-  return out ? launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 1, cus, s, piece)
-             : launch_count_orient_t<false>(p, nv, out, total, groups, by_in, 0, cus, s, piece);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
+                               bool by_in, int cus, hipStream_t s, int64_t piece) {
+  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, by_in, 1, cus, s, piece)
+             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, by_in, 0, cus, s, piece);
 }
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t* out, int64_t* total, int groups, bool by_in,
-                               int cus, hipStream_t s, int64_t piece) {
-  if (!out) return launch_count_orient_t<false>(p, nv, (int32_t*)nullptr, total, groups, by_in, 0, cus, s, piece);
-  return launch_count_orient_t<true>(p, nv, out, total, groups, by_in, 2, cus, s, piece);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
+                               bool by_in, int cus, hipStream_t s, int64_t piece) {
+  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, by_in, 0, cus, s, piece);
+  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, by_in, 2, cus, s, piece);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {
@@ -1011,7 +1017,8 @@ hipError_t launch_orient_t(const Prep& p, int64_t nv, const IT* ind, int cus, hi
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
   hipError_t e = launch_fill_i64(p.dscal + 5, 1, 0, s);
   if (e != hipSuccess || nv <= 0) return e;
-  hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv * 64, 256, (int64_t)cus * 16)), dim3(256), 0, s,
+  // at most one wave per 64 vertices: ceil(nv / 256) workgroups of 4 waves, capped
+  hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
                      (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
                      reinterpret_cast<unsigned long long*>(p.dscal + 5));
   return hipGetLastError();

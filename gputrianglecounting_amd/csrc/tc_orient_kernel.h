@@ -15,7 +15,8 @@
 // streams N+ of the lower end of each arc, on average the shorter list
 // under degree order, but a hub row then has as many partners as the hub
 // has lower neighbours.  Work is therefore handed out in pieces of P
-// consecutive arc positions (counter ctr); prow[b] is the row holding
+// consecutive arc positions (from the counter ctr, or with ONE piece i for
+// wave i, when the grid has a wave for every piece); prow[b] is the row holding
 // position b * P, and a piece walks the rows it overlaps, taking the
 // partners that fall inside it, so a hub row spreads over many waves and a
 // run of short rows shares one.
@@ -116,7 +117,7 @@ __device__ __forceinline__ void out_add(int64_t* out, int64_t i, int64_t v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(out + i), (unsigned long long)v);
 }
 
-template <int AW, int G, bool PF, int WPE, bool PV, typename OT>
+template <int AW, int G, bool PF, int WPE, bool PV, typename OT, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
@@ -134,10 +135,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
   const int64_t M = voff[nv];  // arc positions
+  // ONE: the grid has a wave for every piece (a small graph), and wave i
+  // takes piece i and nothing else: no counter, whose single address would
+  // otherwise serialise a few thousand waves' atomics.
   for (;;) {
     int64_t b = 0;
-    if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
-    b = uniform64(__shfl(b, 0, 64));
+    if (ONE) {
+      b = (int64_t)blockIdx.x * NW + wv;
+    } else {
+      if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
+      b = uniform64(__shfl(b, 0, 64));
+    }
     const int64_t a0 = b * P;
     if (a0 >= M) break;
     const int64_t a1 = a0 + P < M ? a0 + P : M;
@@ -250,6 +258,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       }
       if (t_next >= a1) break;
     }
+    if (ONE) break;  // its one piece done
   }
   if (lane == 0 && total && tri) atomicAdd(total, (unsigned long long)(6 * tri));
 }
