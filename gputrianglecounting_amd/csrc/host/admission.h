@@ -14,6 +14,7 @@
 #include <deque>
 #include <iterator>
 #include <map>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -157,16 +158,22 @@ inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, 
   std::vector<size_t> start(tasks.size() + 1, 0);
   for (size_t i = 0; i < tasks.size(); ++i) start[i + 1] = start[i] + tasks[i].n;
   std::vector<std::thread> pool;
+  pool.reserve((size_t)nt);
   size_t i0 = 0;
-  for (int k = 0; k < nt && i0 < tasks.size(); ++k) {
-    const size_t hi = (total * (size_t)(k + 1)) / (size_t)nt;
-    size_t i1 = i0;
-    while (i1 < tasks.size() && (start[i1] < hi || k == nt - 1)) ++i1;
-    pool.emplace_back([&tasks, i0, i1] {
-      for (size_t i = i0; i < i1; ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
-    });
-    i0 = i1;
+  try {
+    for (int k = 0; k < nt && i0 < tasks.size(); ++k) {
+      const size_t hi = (total * (size_t)(k + 1)) / (size_t)nt;
+      size_t i1 = i0;
+      while (i1 < tasks.size() && (start[i1] < hi || k == nt - 1)) ++i1;
+      pool.emplace_back([&tasks, i0, i1] {
+        for (size_t i = i0; i < i1; ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
+      });
+      i0 = i1;
+    }
+  } catch (const std::system_error&) {
+    // a thread could not start: the tasks not yet handed out are copied below
   }
+  for (size_t i = i0; i < tasks.size(); ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
   for (auto& t : pool) t.join();
   return L;
 }
