@@ -319,11 +319,13 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
 //
 // Waves take 64 vertices at a time from ctr (the grid has at most one wave
 // per 64 vertices, so a small graph does not crowd the counter with idle
-// waves).  A row of at most kShortRow
-// arcs is done by its own lane, four arcs per step (sparse graphs have
-// rows of a few arcs, which would leave most of a wave idle); the longer
-// rows of the 64 are then done one at a time by the whole wave, 256 arcs
-// per step, with ballots placing the kept and the other arcs.
+// waves).  A row of at most kShortRow arcs is done by its own lane, four
+// arcs per step (sparse graphs have rows of a few arcs, which would leave
+// most of a wave idle); the longer rows of the 64 are then done one at a
+// time by the whole wave, 512 arcs per step with the next step's ids
+// already loading, and ballots placing the kept and the other arcs.  The
+// hub rows (hundreds of thousands of arcs, one wave each) are what the
+// pass waits for last, so their steps are made long and overlapped.
 constexpr int64_t kShortRow = 32;
 
 template <typename IT>
@@ -333,22 +335,30 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
   const int64_t dv = e - b;
   const int32_t vl = (int32_t)(v - vb);
   int64_t pf = b, pb = e;  // next front position, one past the next back one
-  for (int64_t k0 = b; k0 < e; k0 += 256) {
-    int32_t wl[4];
+  int32_t wn[8];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t k = k0 + lane + 64 * r;
-      wl[r] = k < e ? (int32_t)ind[k] : -1;
-    }
-    int64_t lo[4], hi[4];
+  for (int r = 0; r < 8; ++r) {
+    const int64_t k = b + lane + 64 * r;
+    wn[r] = k < e ? (int32_t)ind[k] : -1;
+  }
+  for (int64_t k0 = b; k0 < e; k0 += 512) {
+    int32_t wl[8];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 8; ++r) wl[r] = wn[r];
+    int64_t lo[8], hi[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
       const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
       lo[r] = voff[wg];
       hi[r] = voff[wg + 1];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 8; ++r) {  // the next step's ids, in flight behind the degree reads
+      const int64_t k = k0 + 512 + lane + 64 * r;
+      wn[r] = k < e ? (int32_t)ind[k] : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
       const bool valid = wl[r] >= 0;
       const bool keep = valid && orient_keep(dv, vl, hi[r] - lo[r], wl[r]);
       const uint64_t mk = __ballot(keep), mi = __ballot(valid && !keep);
