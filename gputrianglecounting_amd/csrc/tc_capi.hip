@@ -120,7 +120,7 @@ enum class Scheme { kEdges, kList, kOrient, kOrientList };
 Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
 
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, vmark, vcnt, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, vmark, vcnt, total;
   int64_t ecap;
   Scheme sc;
 };
@@ -151,6 +151,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.orow = take(orient ? 16 * (size_t)(nv + 1) : 0);
   L.oind = take(orient ? 4 * (size_t)(nnz + 1) : 0);
   L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2 + tcd::kOrientPieceSlack) : 0);
+  L.huge = take(orient ? 4 * (size_t)(nnz / tcd::kHugeRow + 1) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(counts ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
@@ -202,6 +203,7 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.orow = orient ? reinterpret_cast<int64_t*>(b + L.orow) : nullptr;
   p.oind = orient ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
   p.prow = orient ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
+  p.huge = orient ? reinterpret_cast<int32_t*>(b + L.huge) : nullptr;
   p.vmark = list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
   p.vcnt = counts ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
@@ -387,7 +389,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     // three gets 2 (out[v] = 2 T(v)), the total 6 per triangle.
     // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-    if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
     if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
@@ -427,7 +429,7 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   if (d_out && (e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+  if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
   if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, cfg.oriented == 2, c->cus, s)) != hipSuccess)
@@ -477,7 +479,7 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-  if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+  if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
   if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups, false, c->cus,
@@ -517,7 +519,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     if ((e = tcd::launch_fill_i32(p.vcnt, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-    if ((e = tcd::launch_orient(p, nv, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
     if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, cfg.oriented == 2, c->cus, s)) !=
@@ -582,7 +584,8 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     // (graph-local ids translated through p.vbase) never joins two graphs,
     // so every triangle stays inside its graph.
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-    if ((e = tcd::launch_orient(p, nv_total, d_ind_cat, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    if ((e = tcd::launch_orient(p, nv_total, nnz_total, d_ind_cat, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv_total, nnz_total, true);
     if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, cfg.oriented == 2, c->cus, s,

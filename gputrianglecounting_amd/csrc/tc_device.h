@@ -63,6 +63,10 @@ inline int64_t orient_piece(int64_t nnz, int64_t base, int64_t waves) {
   return p;
 }
 
+// Rows longer than this are listed and oriented first, one row per wave, so
+// that a task of 64 vertices never holds several hub rows.
+constexpr int64_t kHugeRow = 4096;
+
 // Scratch sizing and prep/count launchers (stream-ordered, no host sync).
 struct Prep {
   int64_t* voff;    // [nv + 1]
@@ -75,7 +79,8 @@ struct Prep {
   int32_t* ew;      // [ecap]
   int32_t* ework;   // [ecap]
   int64_t* wpre;    // [ecap + 1]
-  int64_t* dscal;   // [8]: E, W, overflow flag, -, two vertex-chunk counters
+  int64_t* dscal;   // [8]: E, W, overflow flag, -, piece counter, orientation task counter,
+                    // huge rows listed, huge rows taken
   unsigned int* qctr;  // [24]: three launches' per-XCD range queues
   int64_t* scan_tmp;
   int64_t ecap;
@@ -86,6 +91,8 @@ struct Prep {
   int32_t* oind;    // [nnz]
   // row of the first arc of every piece (orient_piece arc positions)
   int32_t* prow;    // [nnz / kOrientPiece + 2 + kOrientPieceSlack]
+  // rows of more than kHugeRow arcs, oriented one per wave before the rest
+  int32_t* huge;    // [nnz / kHugeRow + 1]
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -99,10 +106,11 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 // mode 1: all arcs of an oriented CSR whose endpoints both have out-arcs
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
                               int cus, hipStream_t s);
-// Orientation in place (k_orient_rows) into p.orow / p.oind; ind holds
-// graph-local ids when p.vbase is set.
-hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s);
-hipError_t launch_orient(const Prep& p, int64_t nv, const int64_t* ind, int cus, hipStream_t s);
+// Orientation in place (k_orient_rows) into p.orow / p.oind of a CSR with
+// nnz arcs (the size p was laid out for); ind holds graph-local ids when
+// p.vbase is set.
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant

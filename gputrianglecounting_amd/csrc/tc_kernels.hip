@@ -375,14 +375,53 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
   }
 }
 
+// Rows of more than kHugeRow arcs into huge[0 .. *cnt), one atomic per
+// wave; a CSR whose rows sum to more than the list was sized for (nnz /
+// kHugeRow + 1 = cap entries) lists only the first cap.
+__global__ void k_huge_rows(const int64_t* __restrict__ voff, int64_t nv, int32_t* __restrict__ huge, int64_t cap,
+                            unsigned long long* cnt) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nv; base += step) {  // uniform per workgroup
+    const int64_t u = base + threadIdx.x;
+    const bool h = u < nv && voff[u + 1] - voff[u] > kHugeRow;
+    const uint64_t m = __ballot(h);
+    if (!m) continue;
+    unsigned long long at = 0;
+    if (lane == 0) at = atomicAdd(cnt, (unsigned long long)__popcll(m));
+    const int64_t i = (int64_t)__shfl(at, 0, 64) + __popcll(m & lt);
+    if (h && i < cap) huge[i] = (int32_t)u;
+  }
+}
+
 // IT: the caller's id type (int32 or int64 CSR); ids are below 2^31.
+// First the listed huge rows, one per wave from hctr (they start together
+// and run beside everything else), then tasks of 64 vertices from ctr,
+// whose long-row pass skips the listed rows.
 template <typename IT>
 __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const IT* __restrict__ ind,
                                                      const int32_t* __restrict__ vbase, int64_t nv,
                                                      int32_t* __restrict__ oind, int64_t* __restrict__ orow,
-                                                     unsigned long long* ctr) {
+                                                     unsigned long long* ctr, const int32_t* __restrict__ huge,
+                                                     int64_t cap, const unsigned long long* nhuge,
+                                                     unsigned long long* hctr) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int64_t nh = uniform64((int64_t)*nhuge);
+  if (nh > cap) nh = cap;
+  // The ticket is taken by all lanes at once (lane 0 adds 1, the others 0;
+  // lane 0's result is the ticket), with no branch on the lane: a lane-0
+  // branch here was merged by the compiler with the lane-0 store at the end
+  // of orient_row_wave, leaving lanes 1-63 looping without a new ticket.
+  if (nh > 0) {
+    for (;;) {
+      const int64_t i = uniform64((int64_t)atomicAdd(hctr, lane == 0 ? 1ull : 0ull));
+      if (i >= nh) break;
+      const int64_t v = uniform64(huge[i]);
+      orient_row_wave(voff, ind, oind, orow, v, voff[v], voff[v + 1], vbase ? vbase[v] : 0, lane, lt);
+    }
+  }
   for (;;) {
     int64_t v0 = 0;
     if (lane == 0) v0 = (int64_t)atomicAdd(ctr, 64ull);
@@ -423,7 +462,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       orow[2 * v + 1] = pf;
     }
     // the long rows among the 64, one at a time by the whole wave
-    uint64_t lm = __ballot(mine && e - b > kShortRow);
+    uint64_t lm = __ballot(mine && e - b > kShortRow && e - b <= kHugeRow);
     while (lm) {
       const int l = __builtin_ctzll(lm);
       lm &= lm - 1;
@@ -1023,21 +1062,27 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
 }
 
 template <typename IT>
-hipError_t launch_orient_t(const Prep& p, int64_t nv, const IT* ind, int cus, hipStream_t s) {
+hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind, int cus, hipStream_t s) {
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
-  hipError_t e = launch_fill_i64(p.dscal + 5, 1, 0, s);
+  hipError_t e = launch_fill_i64(p.dscal + 5, 3, 0, s);  // task counter, huge rows listed and taken
   if (e != hipSuccess || nv <= 0) return e;
+  if (!p.huge) return hipErrorInvalidValue;
+  const int64_t cap = nnz / kHugeRow + 1;
+  hipLaunchKernelGGL(k_huge_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, p.huge,
+                     cap, reinterpret_cast<unsigned long long*>(p.dscal + 6));
   // at most one wave per 64 vertices: ceil(nv / 256) workgroups of 4 waves, capped
   hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
                      (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
-                     reinterpret_cast<unsigned long long*>(p.dscal + 5));
+                     reinterpret_cast<unsigned long long*>(p.dscal + 5), (const int32_t*)p.huge, cap,
+                     reinterpret_cast<const unsigned long long*>(p.dscal + 6),
+                     reinterpret_cast<unsigned long long*>(p.dscal + 7));
   return hipGetLastError();
 }
-hipError_t launch_orient(const Prep& p, int64_t nv, const int32_t* ind, int cus, hipStream_t s) {
-  return launch_orient_t(p, nv, ind, cus, s);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s) {
+  return launch_orient_t(p, nv, nnz, ind, cus, s);
 }
-hipError_t launch_orient(const Prep& p, int64_t nv, const int64_t* ind, int cus, hipStream_t s) {
-  return launch_orient_t(p, nv, ind, cus, s);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s) {
+  return launch_orient_t(p, nv, nnz, ind, cus, s);
 }
 
 int occupancy(void (*fn)(CountArgs), int bs) {
