@@ -120,7 +120,7 @@ enum class Scheme { kEdges, kList, kOrient, kOrientList };
 Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
 
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, vmark, vcnt, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, odeg, vmark, vcnt, total;
   int64_t ecap;
   Scheme sc;
 };
@@ -152,6 +152,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.oind = take(orient ? 4 * (size_t)(nnz + 1) : 0);
   L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2 + tcd::kOrientPieceSlack) : 0);
   L.huge = take(orient ? 4 * (size_t)(nnz / tcd::kHugeRow + 1) : 0);
+  L.odeg = take(orient ? 4 * (size_t)(nv + 1) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(counts ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
@@ -204,6 +205,7 @@ tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
   p.oind = orient ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
   p.prow = orient ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
   p.huge = orient ? reinterpret_cast<int32_t*>(b + L.huge) : nullptr;
+  p.odeg = orient ? reinterpret_cast<int32_t*>(b + L.odeg) : nullptr;
   p.vmark = list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
   p.vcnt = counts ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;

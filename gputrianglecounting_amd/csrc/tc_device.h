@@ -93,6 +93,7 @@ struct Prep {
   int32_t* prow;    // [nnz / kOrientPiece + 2 + kOrientPieceSlack]
   // rows of more than kHugeRow arcs, oriented one per wave before the rest
   int32_t* huge;    // [nnz / kHugeRow + 1]
+  int32_t* odeg;    // [nv + 1]: every vertex's degree, read by the orientation
 };
 
 size_t scan_tmp_elems(int64_t n_cap);

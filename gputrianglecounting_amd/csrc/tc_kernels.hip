@@ -329,7 +329,7 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
 constexpr int64_t kShortRow = 32;
 
 template <typename IT>
-__device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff, const IT* __restrict__ ind,
+__device__ __forceinline__ void orient_row_wave(const int32_t* __restrict__ deg, const IT* __restrict__ ind,
                                                 int32_t* __restrict__ oind, int64_t* __restrict__ orow, int64_t v,
                                                 int64_t b, int64_t e, int32_t vb, int lane, uint64_t lt) {
   const int64_t dv = e - b;
@@ -345,13 +345,9 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
     int32_t wl[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) wl[r] = wn[r];
-    int64_t lo[8], hi[8];
+    int32_t dg[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
-      lo[r] = voff[wg];
-      hi[r] = voff[wg + 1];
-    }
+    for (int r = 0; r < 8; ++r) dg[r] = deg[(int64_t)vb + (wl[r] < 0 ? 0 : wl[r])];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {  // the next step's ids, in flight behind the degree reads
       const int64_t k = k0 + 512 + lane + 64 * r;
@@ -360,7 +356,7 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const bool valid = wl[r] >= 0;
-      const bool keep = valid && orient_keep(dv, vl, hi[r] - lo[r], wl[r]);
+      const bool keep = valid && orient_keep(dv, vl, dg[r], wl[r]);
       const uint64_t mk = __ballot(keep), mi = __ballot(valid && !keep);
       const int32_t wg = (int32_t)((int64_t)vb + wl[r]);
       if (keep) oind[pf + __popcll(mk & lt)] = wg;
@@ -375,17 +371,21 @@ __device__ __forceinline__ void orient_row_wave(const int64_t* __restrict__ voff
   }
 }
 
-// Rows of more than kHugeRow arcs into huge[0 .. *cnt), one atomic per
+// Every vertex's degree into deg (4 bytes: the orientation's random
+// reads then touch half the footprint of the 16-byte offset pairs), and
+// the rows of more than kHugeRow arcs into huge[0 .. *cnt), one atomic per
 // wave; a CSR whose rows sum to more than the list was sized for (nnz /
 // kHugeRow + 1 = cap entries) lists only the first cap.
 __global__ void k_huge_rows(const int64_t* __restrict__ voff, int64_t nv, int32_t* __restrict__ huge, int64_t cap,
-                            unsigned long long* cnt) {
+                            unsigned long long* cnt, int32_t* __restrict__ deg) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nv; base += step) {  // uniform per workgroup
     const int64_t u = base + threadIdx.x;
-    const bool h = u < nv && voff[u + 1] - voff[u] > kHugeRow;
+    const int64_t d = u < nv ? voff[u + 1] - voff[u] : 0;
+    if (u < nv) deg[u] = (int32_t)d;  // a simple graph's degree is below nv < 2^31
+    const bool h = d > kHugeRow;
     const uint64_t m = __ballot(h);
     if (!m) continue;
     unsigned long long at = 0;
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
                                                      int32_t* __restrict__ oind, int64_t* __restrict__ orow,
                                                      unsigned long long* ctr, const int32_t* __restrict__ huge,
                                                      int64_t cap, const unsigned long long* nhuge,
-                                                     unsigned long long* hctr) {
+                                                     unsigned long long* hctr, const int32_t* __restrict__ deg) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   int64_t nh = uniform64((int64_t)*nhuge);
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       const int64_t i = uniform64((int64_t)atomicAdd(hctr, lane == 0 ? 1ull : 0ull));
       if (i >= nh) break;
       const int64_t v = uniform64(huge[i]);
-      orient_row_wave(voff, ind, oind, orow, v, voff[v], voff[v + 1], vbase ? vbase[v] : 0, lane, lt);
+      orient_row_wave(deg, ind, oind, orow, v, voff[v], voff[v + 1], vbase ? vbase[v] : 0, lane, lt);
     }
   }
   for (;;) {
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int64_t wg = (int64_t)vb + (wl[r] < 0 ? 0 : wl[r]);
-          dw[r] = voff[wg + 1] - voff[wg];
+          dw[r] = deg[wg];
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
                          ((int64_t)__builtin_amdgcn_readlane((int32_t)((uint64_t)b >> 32), l) << 32);
       const int64_t el = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(uint64_t)e, l) |
                          ((int64_t)__builtin_amdgcn_readlane((int32_t)((uint64_t)e >> 32), l) << 32);
-      orient_row_wave(voff, ind, oind, orow, v0 + l, bl, el, __builtin_amdgcn_readlane(vb, l), lane, lt);
+      orient_row_wave(deg, ind, oind, orow, v0 + l, bl, el, __builtin_amdgcn_readlane(vb, l), lane, lt);
     }
   }
 }
@@ -1066,16 +1066,16 @@ hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
   hipError_t e = launch_fill_i64(p.dscal + 5, 3, 0, s);  // task counter, huge rows listed and taken
   if (e != hipSuccess || nv <= 0) return e;
-  if (!p.huge) return hipErrorInvalidValue;
+  if (!p.huge || !p.odeg) return hipErrorInvalidValue;
   const int64_t cap = nnz / kHugeRow + 1;
   hipLaunchKernelGGL(k_huge_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, p.huge,
-                     cap, reinterpret_cast<unsigned long long*>(p.dscal + 6));
+                     cap, reinterpret_cast<unsigned long long*>(p.dscal + 6), p.odeg);
   // at most one wave per 64 vertices: ceil(nv / 256) workgroups of 4 waves, capped
   hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
                      (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
                      reinterpret_cast<unsigned long long*>(p.dscal + 5), (const int32_t*)p.huge, cap,
                      reinterpret_cast<const unsigned long long*>(p.dscal + 6),
-                     reinterpret_cast<unsigned long long*>(p.dscal + 7));
+                     reinterpret_cast<unsigned long long*>(p.dscal + 7), (const int32_t*)p.odeg);
   return hipGetLastError();
 }
 hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s) {
