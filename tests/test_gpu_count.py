@@ -436,6 +436,47 @@ def test_global_count_mode(counter):
         assert int(tot.cpu().item()) == rtot, g.name
 
 
+def _hub_graphs():
+    """Graphs with rows of more than 4096 arcs, which the orientation lists
+    and hands out first, one row per wave: two hubs of 6001 arcs, and 50
+    hubs of 4149 arcs (a K_50 whose vertices all join the same 4100 leaves,
+    so every leaf closes C(50, 2) triangles)."""
+    k, m = 50, 4100
+    a = np.repeat(np.arange(k), m)
+    b = np.tile(np.arange(k, k + m), k)
+    ii, jj = np.triu_indices(k, 1)
+    hubs = G.from_edges(k + m, np.concatenate([a, ii]), np.concatenate([b, jj]), f"hubs{k}x{m}")
+    return [GR.two_hubs(6000), hubs]
+
+
+def test_hub_rows_every_entry_point(counter):
+    """Hub rows through every oriented entry point: per-vertex, total only,
+    a vertex list of every vertex (the oriented count and a gather) and a
+    batch in which they sit between ordinary graphs."""
+    import torch
+
+    graphs = _hub_graphs()
+    for g in graphs:
+        _check(counter, g)
+        ref, rtot = _ref(g)
+        off, ind = counter.to_device(g)
+        assert int(counter.count_total(off, ind).cpu().item()) == rtot, g.name
+        vl = np.arange(g.n, dtype=np.int32)[::-1].copy()
+        out, tot = counter.count_vertices(off, ind, torch.from_numpy(vl).to(off.device))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), ref[vl], err_msg=g.name)
+        assert int(tot.item()) == rtot, g.name
+    b = G.pack([G.rmat(12, 16, seed=2)] + graphs + [GR.complete(60), GR.empty(4)])
+    print(f"case hub batch graphs={b.count}", flush=True)
+    out, totals = counter.count_batch(b)
+    for gi in range(b.count):
+        gr = b.graph(gi)
+        ref, rtot = oracle.per_vertex(gr.off, gr.ind)
+        vb = b.v_base(gi)
+        np.testing.assert_array_equal(out[vb: vb + gr.n], ref, err_msg=f"graph {gi}")
+        assert totals[gi] == rtot
+
+
 def test_batch_equals_alone(counter):
     b = G.gen_batch(G=96, n_lo=16, n_hi=3000, seed=3)
     out, totals = counter.count_batch(b)
