@@ -65,14 +65,15 @@ typedef struct tc_ctx tc_ctx;
  *                  least probe_ratio times longer than A is merged from A's
  *                  side, each A value tested against a 16-element chunk of B
  *                  in registers (default 4; negative: never)
- *   oriented       algo 0 (tc_count_i32, tc_batch_count_i32): 1 (default) =
+ *   oriented       algo 0 (tc_count_i32, tc_batch_count_i32): 1 =
  *                  intersect the degree-ordered orientation, built in place
  *                  of the input rows, each triangle found once and its three
  *                  vertices credited; 2 = the same, but a row takes its
  *                  in-neighbours as partners (streams the lower end's row of
- *                  each arc); -1 = every undirected edge's full lists (the
- *                  reference's scheme, the edge-list kernels).  Same counts
- *                  either way; other algos always use the full lists.
+ *                  each arc); 3 (default) = automatic: 2 for batches, 1 for
+ *                  single graphs; -1 = every undirected edge's full lists
+ *                  (the reference's scheme, the edge-list kernels).  Same
+ *                  counts either way; other algos always use the full lists.
  *                  tc_last_edge_work reports (0, 0) for the oriented scheme,
  *                  which builds no edge list.
  *   row_groups     oriented scheme: partners of a row a wave streams at once,

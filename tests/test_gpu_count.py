@@ -477,9 +477,14 @@ def test_hub_rows_every_entry_point(counter):
         assert totals[gi] == rtot
 
 
-def test_batch_equals_alone(counter):
+@pytest.mark.parametrize("cfg", [None, dict(oriented=1), dict(oriented=2), dict(oriented=-1)], ids=str)
+def test_batch_equals_alone(counter, cfg):
+    """Every graph of a batch gets the counts it gets alone: the default
+    (in-neighbour partners for a batch), each partner side explicitly, and
+    the edge scheme."""
     b = G.gen_batch(G=96, n_lo=16, n_hi=3000, seed=3)
-    out, totals = counter.count_batch(b)
+    print(f"case batch cfg={cfg}", flush=True)
+    out, totals = counter.count_batch(b, cfg=cfg)
     for g in range(b.count):
         gr = b.graph(g)
         ref, rtot = oracle.per_vertex(gr.off, gr.ind)
