@@ -1,11 +1,15 @@
 """Benchmark: per-vertex triangle counting on one MI355X.
 
 Default workload: R-MAT scale 24, edge factor 16 (a=.57 b=.19 c=.19),
-seed 1, simple symmetric int32 CSR (~520M arcs), counted by the two-level
-Merge Path kernel.  A step is one full tc_count_i32 call over the resident
-graph: edge-list prep, the count kernel and the int64 total.  The metric is
-undirected edges / step time (edge-intersections/s); the roofline entry is
-for the count kernel, timed with HIP events on its launch stream.
+seed 1, simple symmetric int32 CSR (~520M arcs), counted by the default
+configuration: the degree orientation written in place of the input rows
+(k_huge_rows, k_orient_rows), then k_count_orient.  A step is one full
+tc_count_i32 call over the resident graph: the orientation, the count
+kernel and the int64 total.  The metric is undirected edges / step time
+(edge-intersections/s); the roofline entry is for the count kernel, timed
+with HIP events on its launch stream.  The default run adds the batched
+comparison: 4096 mixed graphs counted as one batch against one launch
+sequence per graph.
 
 The CPU baseline runs the reference's own intersection (oracle/_ref, or the
 C restatement when that is absent) on a bounded uniform sample of arcs of
