@@ -52,7 +52,7 @@ tc_cfg builtin_default() {
   c.algo = 0;
   c.search_ratio = 64;
   c.probe_ratio = 4;
-  c.oriented = 3;     // automatic (orient_by_in)
+  c.oriented = 3;     // automatic (orient_side)
   c.row_groups = -1;  // automatic (orient_groups)
   return c;
 }
@@ -119,11 +119,21 @@ enum class Scheme { kEdges, kList, kOrient, kOrientList };
 
 Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
 
-// Rows take their in-neighbours as partners (streaming the lower end's row
-// of each arc) with oriented = 2, and with the automatic 3 in a batch: the
-// mixed batch of 4096 graphs counts in 33.1 ms that way against 36.6 ms,
-// while R-MAT s24 takes 393 ms against 207 ms with out-neighbour partners.
-bool orient_by_in(const tc_cfg& cfg, bool batch) { return cfg.oriented == 2 || (cfg.oriented == 3 && batch); }
+// Partner side of the oriented count (launch_count_orient's side): rows take
+// their in-neighbours as partners (streaming the lower end's row of each
+// arc) with oriented = 2 and out-neighbours with 1.  The automatic 3 takes
+// in-neighbours for batches and for single graphs below 2^24 arcs, and for
+// larger single graphs unless their orientation lists hub rows (more than
+// 4096 arcs, decided on the device).  Measured, in/out count time: the mixed
+// batch 33.1/36.6 ms, R-MAT s14 0.09/0.18, s18 1.84/2.33, RGG n=4M 1.73/1.97,
+// ER n=4M 1.86/1.98; but R-MAT s20 7.32/6.27 and s24 393/207 ms, where the
+// hub-side lists streamed with out-neighbour partners stay in the caches.
+constexpr int64_t kSideArcs = int64_t(1) << 24;
+int orient_side(const tc_cfg& cfg, bool batch, int64_t nnz) {
+  if (cfg.oriented == 2) return 1;
+  if (cfg.oriented != 3) return 0;
+  return batch || nnz < kSideArcs ? 1 : 2;
+}
 
 struct Layout {
   size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, odeg, vmark, vcnt, total;
@@ -400,7 +410,7 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
     if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_by_in(cfg, false), c->cus,
+    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
                                       s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
@@ -441,7 +451,7 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_by_in(cfg, false), c->cus,
+  if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
                                       s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
@@ -492,8 +502,8 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups, false, c->cus,
-                                    s)) != hipSuccess)
+  if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups,
+                                    orient_side(cfg, false, nnz), c->cus, s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -532,7 +542,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, orient_by_in(cfg, false), c->cus, s)) !=
+    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, orient_side(cfg, false, nnz), c->cus, s)) !=
         hipSuccess)
       return hip_fail(c, e, "count kernel launch");
     c->last_dscal = p.dscal;
@@ -598,8 +608,8 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
       return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv_total, nnz_total, true);
-    const bool by_in = orient_by_in(cfg, true);
-    if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, by_in, c->cus, s,
+    const int side = orient_side(cfg, true, nnz_total);
+    if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, side, c->cus, s,
                                       tcd::kOrientPieceBatch)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {

@@ -127,14 +127,15 @@ hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
 // Per-vertex counts (out; nullptr: the total only) over the orientation in
 // p.orow / p.oind (k_count_orient) of nnz arcs.  groups: arcs of a row
-// streamed at a time per wave (1, 2, 4, 8, 16 or 32).  by_in: rows are the
-// higher vertex of each arc (partners are its in-neighbours) instead of the
-// lower one (partners are its out-neighbours).  piece: the base piece size
-// (orient_piece picks the one used).
+// streamed at a time per wave (1, 2, 4, 8, 16 or 32).  side 1: rows are the
+// higher vertex of each arc (partners are its in-neighbours); 0: the lower
+// one (partners are its out-neighbours); 2: 1 unless the orientation of the
+// same call listed rows of more than kHugeRow arcs (p.dscal[6]), else 0.
+// piece: the base piece size (orient_piece picks the one used).
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               bool by_in, int cus, hipStream_t s, int64_t piece = kOrientPiece);
+                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece);
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               bool by_in, int cus, hipStream_t s, int64_t piece = kOrientPiece);
+                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

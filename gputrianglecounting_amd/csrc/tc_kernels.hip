@@ -1019,7 +1019,7 @@ int resident_orient_wgs(int groups, int kind, int cus) {
 
 template <bool PV, typename OT>
 hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total, int groups,
-                                 bool by_in, int kind, int cus, hipStream_t s, int64_t base) {
+                                 int side, int kind, int cus, hipStream_t s, int64_t base) {
   if (base < kOrientPieceMin || 2 * base >= 65536 || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
@@ -1034,18 +1034,18 @@ hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out
   const OrientFn<OT> fn = 4 * (int64_t)nwg * piece >= nnz ? orient_fn<PV, OT, true>(groups) : orient_fn<PV, OT>(groups);
   hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
                      (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total),
-                     ctr, by_in ? 1 : 0, (int32_t)piece);
+                     ctr, side, (int32_t)piece);
   return hipGetLastError();
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               bool by_in, int cus, hipStream_t s, int64_t piece) {
-  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, by_in, 1, cus, s, piece)
-             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, by_in, 0, cus, s, piece);
+                               int side, int cus, hipStream_t s, int64_t piece) {
+  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 1, cus, s, piece)
+             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, side, 0, cus, s, piece);
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               bool by_in, int cus, hipStream_t s, int64_t piece) {
-  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, by_in, 0, cus, s, piece);
-  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, by_in, 2, cus, s, piece);
+                               int side, int cus, hipStream_t s, int64_t piece) {
+  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, side, 0, cus, s, piece);
+  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 2, cus, s, piece);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {

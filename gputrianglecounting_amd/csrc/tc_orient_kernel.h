@@ -121,7 +121,7 @@ template <int AW, int G, bool PF, int WPE, bool PV, typename OT, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
-    unsigned long long* ctr, int by_in, int32_t P) {
+    unsigned long long* ctr, int side, int32_t P) {
   constexpr int NW = 4;
   constexpr int CH = 8;
   constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
@@ -135,6 +135,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
   const int64_t M = voff[nv];  // arc positions
+  // side 0: partners are out-neighbours, 1: in-neighbours, 2: in-neighbours
+  // unless the orientation listed rows of more than 4096 arcs (their count
+  // sits two words after the piece counter)
+  const bool by_in = side == 2 ? uniform64((int64_t)ctr[2]) == 0 : side != 0;
   // ONE: the grid has a wave for every piece (a small graph), and wave i
   // takes piece i and nothing else: no counter, whose single address would
   // otherwise serialise a few thousand waves' atomics.

@@ -70,8 +70,10 @@ typedef struct tc_ctx tc_ctx;
  *                  of the input rows, each triangle found once and its three
  *                  vertices credited; 2 = the same, but a row takes its
  *                  in-neighbours as partners (streams the lower end's row of
- *                  each arc); 3 (default) = automatic: 2 for batches, 1 for
- *                  single graphs; -1 = every undirected edge's full lists
+ *                  each arc); 3 (default) = automatic: 2 for batches and
+ *                  for graphs below 2^24 arcs, and for larger graphs unless
+ *                  they have rows of more than 4096 arcs (then 1, decided
+ *                  on the device); -1 = every undirected edge's full lists
  *                  (the reference's scheme, the edge-list kernels).  Same
  *                  counts either way; other algos always use the full lists.
  *                  tc_last_edge_work reports (0, 0) for the oriented scheme,
