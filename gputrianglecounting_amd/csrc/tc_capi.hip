@@ -17,7 +17,7 @@ struct tc_ctx {
   int device = 0;
   int cus = 256;
   std::string err;
-  void* scratch = nullptr;
+  void* pool = nullptr;  // device scratch owned by the context (grown by ensure)
   size_t cap = 0;
   tc_cfg def{};
   bool timing = false;
@@ -182,26 +182,26 @@ size_t layout_bound(int64_t nv, int64_t nnz, bool batch) {
 
 int ensure(tc_ctx* c, size_t bytes) {
   if (bytes <= c->cap) return TC_OK;
-  if (c->scratch) {
-    hipError_t e = hipFree(c->scratch);
-    c->scratch = nullptr;
+  if (c->pool) {
+    hipError_t e = hipFree(c->pool);
+    c->pool = nullptr;
     c->cap = 0;
     c->last_dscal = nullptr;
     if (e != hipSuccess) return hip_fail(c, e, "hipFree");
   }
   size_t want = bytes + bytes / 8;
-  hipError_t e = hipMalloc(&c->scratch, want);
+  hipError_t e = hipMalloc(&c->pool, want);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    c->scratch = nullptr;
+    c->pool = nullptr;
     return set_err(c, TC_ENOMEM, "device scratch of " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
   }
   c->cap = want;
   return TC_OK;
 }
 
-tcd::Prep make_prep(tc_ctx* c, const Layout& L, bool batch) {
-  char* b = static_cast<char*>(c->scratch);
+tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
+  char* b = static_cast<char*>(base);
   tcd::Prep p{};
   p.voff = reinterpret_cast<int64_t*>(b + L.voff);
   p.vbase = batch ? reinterpret_cast<int32_t*>(b + L.vbase) : nullptr;
@@ -268,6 +268,43 @@ int run_count(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int vi, const in
   return TC_OK;
 }
 
+// Per-vertex count of an int32 CSR (tc_count_i32's launches) with its
+// scratch at base, laid out as L for cfg's scheme.  Allocates nothing.
+int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, int32_t nv, int64_t nnz,
+            const int32_t* d_off, const int32_t* d_ind, int32_t* d_out, int64_t* d_total, hipStream_t s) {
+  hipError_t e;
+  int rc = TC_OK;
+  tcd::Prep p = make_prep(base, L, false);
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
+  if (cfg.oriented) {
+    // Degree-ordered orientation: triangle {a, b, c} with (d, id) order
+    // a < b < c is found once, at arc a -> b with match c; every one of the
+    // three gets 2 (out[v] = 2 T(v)), the total 6 per triangle.
+    // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
+    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
+    record(c, 1, s);
+    const int groups = orient_groups(cfg, nv, nnz);
+    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
+                                      s)) != hipSuccess)
+      return hip_fail(c, e, "count kernel launch");
+  } else {
+    if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "edge list");
+    record(c, 1, s);
+    rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
+  }
+  if (rc) return rc;
+  c->last_dscal = p.dscal;
+  record(c, 2, s);
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
 std::mutex g_shim_mu;
 tc_ctx* g_shim_ctx[64] = {nullptr};
 
@@ -303,7 +340,7 @@ int tc_ctx_destroy(tc_ctx* c) {
   int dev0 = 0;
   (void)hipGetDevice(&dev0);
   (void)hipSetDevice(c->device);
-  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->pool) (void)hipFree(c->pool);
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipSetDevice(dev0);
@@ -374,7 +411,7 @@ int tc_copy_i32(tc_ctx* c, int32_t* d_dst, const int32_t* d_src, int64_t n, void
 
 int tc_last_edge_work(tc_ctx* c, int64_t out[2]) {
   if (!c || !out) return TC_EINVAL;
-  if (!c->scratch || !c->last_dscal) return set_err(c, TC_EINVAL, "no count has run on this context");
+  if (!c->last_dscal) return set_err(c, TC_EINVAL, "no count has run on this context");
   hipError_t e = hipSetDevice(c->device);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(out, c->last_dscal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost);
@@ -390,41 +427,12 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   tc_cfg cfg;
   const int vi = resolve(c, cfg_in, &cfg);
   if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
-  hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   const Layout L = layout(nv, nnz, false, scheme_of(cfg));
   int rc = ensure(c, L.total);
   if (rc) return rc;
-  tcd::Prep p = make_prep(c, L, false);
-  record(c, 0, s);
-  if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-  if (cfg.oriented) {
-    // Degree-ordered orientation: triangle {a, b, c} with (d, id) order
-    // a < b < c is found once, at arc a -> b with match c; every one of the
-    // three gets 2 (out[v] = 2 T(v)), the total 6 per triangle.
-    // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
-    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-    if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-    record(c, 1, s);
-    const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
-                                      s)) != hipSuccess)
-      return hip_fail(c, e, "count kernel launch");
-  } else {
-    if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
-      return hip_fail(c, e, "edge list");
-    record(c, 1, s);
-    rc = run_count(c, p, cfg, vi, d_ind, d_out, d_total, 2, s);
-  }
-  if (rc) return rc;
-  c->last_dscal = p.dscal;
-  record(c, 2, s);
-  record(c, 3, s);
-  c->ev_recorded = c->timing;
-  return TC_OK;
+  return count32(c, c->pool, L, cfg, vi, nv, nnz, d_off, d_ind, d_out, d_total, static_cast<hipStream_t>(stream));
 }
 
 namespace {
@@ -442,7 +450,7 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   const Layout L = layout(nv, nnz, false, Scheme::kOrient);
   int rc = ensure(c, L.total);
   if (rc) return rc;
-  tcd::Prep p = make_prep(c, L, false);
+  tcd::Prep p = make_prep(c->pool, L, false);
   p.voff = const_cast<int64_t*>(d_off);  // the caller's offsets are already int64; the kernels only read them
   record(c, 0, s);
   if (d_out && (e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
@@ -491,7 +499,7 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   const Layout L = layout(nv, nnz, false, Scheme::kOrient);
   int rc = ensure(c, L.total);
   if (rc) return rc;
-  tcd::Prep p = make_prep(c, L, false);
+  tcd::Prep p = make_prep(c->pool, L, false);
   record(c, 0, s);
   // The degree-ordered orientation finds every triangle once; without
   // per-vertex output the count kernel keeps only the total (6 per
@@ -533,7 +541,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     const Layout L = layout(nv, nnz, false, Scheme::kOrientList);
     int rc = ensure(c, L.total);
     if (rc) return rc;
-    tcd::Prep p = make_prep(c, L, false);
+    tcd::Prep p = make_prep(c->pool, L, false);
     record(c, 0, s);
     if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_fill_i32(p.vcnt, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
@@ -556,7 +564,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   const Layout L = layout(nv, nnz, false, Scheme::kList);
   int rc = ensure(c, L.total);
   if (rc) return rc;
-  tcd::Prep p = make_prep(c, L, false);
+  tcd::Prep p = make_prep(c->pool, L, false);
   record(c, 0, s);
   if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_fill_i32(p.vmark, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
@@ -594,7 +602,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
   const Layout L = layout(nv_total, nnz_total, true, scheme_of(cfg));
   int rc = ensure(c, L.total);
   if (rc) return rc;
-  tcd::Prep p = make_prep(c, L, true);
+  tcd::Prep p = make_prep(c->pool, L, true);
   record(c, 0, s);
   if ((e = tcd::launch_fill_i32(d_out_cat, nv_total, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_voff_batch(d_tab, G, d_off_cat, nv_total, p.voff, p.vbase, s)) != hipSuccess)
@@ -647,7 +655,7 @@ int tc_validate_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, co
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   int rc = ensure(c, std::max<size_t>(c->cap, 256));
   if (rc) return rc;
-  int32_t* flag = static_cast<int32_t*>(c->scratch);
+  int32_t* flag = static_cast<int32_t*>(c->pool);
   if ((e = tcd::launch_fill_i32(flag, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_validate(d_off, d_ind, nv, nnz, flag, c->cus, s)) != hipSuccess) return hip_fail(c, e, "validate");
   int32_t h = 0;
