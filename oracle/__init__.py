@@ -46,6 +46,8 @@ def lib():
         L.oracle_per_vertex_omp.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I64P, C.c_int32]
         L.oracle_count_arc_sample.restype = C.c_int64
         L.oracle_count_arc_sample.argtypes = [_I32P, _I32P, _I32P, _I64P, C.c_int64]
+        L.oracle_vertex_list_omp.restype = None
+        L.oracle_vertex_list_omp.argtypes = [C.c_int32, _I32P, _I64P, _I32P, _I32P, _I64P, _I64P, C.c_int32]
         L.oracle_lcc.restype = None
         L.oracle_lcc.argtypes = [C.c_int32, _I32P, _I32P, C.POINTER(C.c_double)]
         _o = L
@@ -155,6 +157,21 @@ def count_vertex_list(vertices, off, ind):
     tot = np.zeros(1, np.int64)
     lib().oracle_count_vertex_list(vl.shape[0], _p32(vl), _p32(off), _p32(ind), _p64(tri), _p64(tot))
     return tri[:narcs], int(tot[0])
+
+
+def vertex_list_omp(vertices, off, ind, threads: int = 16):
+    """Per-list-position row sums of the vertex-list overload
+    (clusteringCount.cpp:125-155) with the listed rows' arcs spread over
+    ``threads`` OpenMP threads (int64 per position), and their total."""
+    off, ind, vl = _c(off, np.int32), _c(ind, np.int32), _c(vertices, np.int32)
+    lens = off[vl.astype(np.int64) + 1].astype(np.int64) - off[vl].astype(np.int64)
+    pre = np.zeros(vl.shape[0] + 1, np.int64)
+    np.cumsum(lens, out=pre[1:])
+    out = np.zeros(max(vl.shape[0], 1), np.int64)
+    tot = np.zeros(1, np.int64)
+    lib().oracle_vertex_list_omp(vl.shape[0], _p32(vl), _p64(pre), _p32(off), _p32(ind), _p64(out), _p64(tot),
+                                 int(threads))
+    return out[: vl.shape[0]], int(tot[0])
 
 
 def list_sums(vertices, off, tri):

@@ -20,6 +20,9 @@
  *       per source vertex (src/clusteringCount.cu:331-357), which is 2*T(v)
  *       on a simple graph.
  *   oracle_per_vertex_omp        same numbers, rows split over OpenMP threads.
+ *   oracle_vertex_list_omp       the vertex-list overload's per-position row
+ *       sums with the listed rows' arcs split over OpenMP threads (the
+ *       sampled check of graphs too large for a full CPU count).
  *   oracle_count_arc_sample      the per-arc intersection of clusteringCount.cpp:57-68
  *       on a caller-chosen arc list (bounded CPU baseline).
  *   oracle_lcc                   local clustering coefficient
@@ -163,4 +166,39 @@ void oracle_lcc(int32_t nv, const int32_t* off, const int32_t* out, double* lcc)
     int64_t d = off[v + 1] - off[v];
     lcc[v] = d < 2 ? 0.0 : (double)out[v] / (double)(d * (d - 1));
   }
+}
+
+/* clusteringCount.cpp:125-155 with the work of the listed rows spread over
+ * OpenMP threads: out[s] = the sum over the arcs of row vertexList[s] of
+ * count_intersectionsReg (the row sum of the reference's triNE for that
+ * source, 2*T(v) on a simple graph), *total = the sum of out.  pre[s] is
+ * the caller's exclusive prefix of the listed rows' lengths (pre[count] =
+ * arcs of all listed rows); the arcs are dealt to the threads in chunks,
+ * so one hub row is shared by all of them.  Pinned against
+ * oracle_count_vertex_list and the compiled reference on small graphs. */
+void oracle_vertex_list_omp(int32_t count, const int32_t* vertexList, const int64_t* pre, const int32_t* off,
+                            const int32_t* ind, int64_t* out, int64_t* total, int32_t threads) {
+  const int64_t arcs = count > 0 ? pre[count] : 0;
+  int64_t sum = 0;
+  for (int32_t s = 0; s < count; ++s) out[s] = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : sum) num_threads(threads)
+  for (int64_t i = 0; i < arcs; ++i) {
+    int32_t lo = 0, hi = count; /* last s with pre[s] <= i */
+    while (hi - lo > 1) {
+      int32_t mid = lo + (hi - lo) / 2;
+      if (pre[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    const int32_t src = vertexList[lo];
+    const int32_t srcLen = off[src + 1] - off[src];
+    const int32_t dest = ind[off[src] + (i - pre[lo])];
+    const int32_t destLen = off[dest + 1] - off[dest];
+    const int64_t c = oracle_count_intersections(src, srcLen, ind + off[src], dest, destLen, ind + off[dest]);
+    if (c) {
+#pragma omp atomic
+      out[lo] += c;
+      sum += c;
+    }
+  }
+  *total = sum;
 }

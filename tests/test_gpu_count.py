@@ -141,8 +141,11 @@ def test_tied_hubs(counter):
 def test_full_size_configs(counter):
     """The parity configurations at full size: R-MAT s20 ef16 and RGG
     n=4,194,304 (mean degree 6) against the oracle (OpenMP), bit for bit,
-    on the oriented and the edge scheme."""
+    on the oriented and the edge scheme, and their FP64 local clustering
+    coefficients bit for bit against the oracle's out/(d(d-1))."""
     import os
+
+    import torch
 
     threads = max(1, min(16, os.cpu_count() or 1))
     for g in [G.geometric(4194304, 6.0, seed=1), G.rmat(20, 16, seed=1)]:
@@ -153,6 +156,54 @@ def test_full_size_configs(counter):
             out, tot = counter.count_graph(g, cfg=cfg)
             np.testing.assert_array_equal(out, ref)
             assert tot == rtot
+        off, ind = counter.to_device(g)
+        out, _ = counter.count(off, ind)
+        lcc = counter.lcc(off, out)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(lcc.cpu().numpy().view(np.int64), oracle.lcc(g.off, ref).view(np.int64))
+
+
+def test_rmat_s24_sampled(counter):
+    """R-MAT scale 24, edge factor 16, seed 1 (the benchmark graph: 16.8M
+    vertices, 520.8M arcs).  A full CPU count takes hours, so the per-vertex
+    counts are checked bit for bit against the oracle's vertex-list row sums
+    (clusteringCount.cpp:125-155, OpenMP) on the 16 highest-degree vertices
+    and 2^16 seeded uniform vertices, their FP64 LCC bit for bit, and every
+    vertex against the size-independent properties: out[v] even, 0 when
+    d(v) < 2, at most d(d-1); the int64 total = the sum of out = 6T, equal
+    to the global-count mode's."""
+    import os
+
+    import torch
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    g = G.rmat(24, 16, seed=1)
+    print(f"case R-MAT s24 n={g.n} nnz={g.nnz}", flush=True)
+    off, ind = counter.to_device(g)
+    out, total = counter.count(off, ind)
+    lcc = counter.lcc(off, out)
+    tot_global = counter.count_total(off, ind)
+    torch.cuda.synchronize()
+    h = out.cpu().numpy().astype(np.int64)
+    hl = lcc.cpu().numpy()
+    tot, totg = int(total.cpu().item()), int(tot_global.cpu().item())
+    del off, ind, out, lcc
+    d = g.degrees()
+    assert (h >= 0).all() and (h % 2 == 0).all() and (h <= d * (d - 1)).all() and (h[d < 2] == 0).all()
+    assert tot == int(h.sum()) == totg and tot % 6 == 0
+    hubs = np.argsort(-d, kind="stable")[:16]
+    rng = np.random.default_rng(24)
+    vl = np.concatenate([hubs, rng.integers(0, g.n, 1 << 16)]).astype(np.int32)
+    print(f"case R-MAT s24: oracle on {vl.shape[0]} listed vertices, {threads} threads", flush=True)
+    ref, rtot = oracle.vertex_list_omp(vl, g.off, g.ind, threads=threads)
+    np.testing.assert_array_equal(h[vl], ref)
+    assert int(h[vl].sum()) == rtot
+    dv = d[vl]
+    off_s = np.zeros(vl.shape[0] + 1, np.int64)
+    np.cumsum(dv, out=off_s[1:])
+    assert off_s[-1] < 2 ** 31
+    exp = oracle.lcc(off_s.astype(np.int32), ref.astype(np.int32))
+    np.testing.assert_array_equal(hl[vl].view(np.int64), exp.view(np.int64))
 
 
 def test_oriented_only_for_algo_0(counter):
