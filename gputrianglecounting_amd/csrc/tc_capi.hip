@@ -13,6 +13,14 @@
 #include "tc.h"
 #include "tc_device.h"
 
+// Scratch of tc_count_noalloc_i32: one copy of this array on every GPU the
+// library runs on, held until the process ends.  In host code `scratch` is
+// only the host-side handle of the symbol, never a device address: the
+// device address is taken once per device (noalloc_setup, from
+// tc_ctx_create with that device current) and only that address is passed
+// to kernels.
+static __device__ __align__(256) unsigned char scratch[TC_NOALLOC_SCRATCH_BYTES];
+
 struct tc_ctx {
   int device = 0;
   int cus = 256;
@@ -308,6 +316,39 @@ int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, i
 std::mutex g_shim_mu;
 tc_ctx* g_shim_ctx[64] = {nullptr};
 
+// Device address of `scratch` per device (nullptr until resolved) and the
+// error of a failed lookup.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {nullptr};
+hipError_t g_noalloc_err[kMaxDevices] = {};
+
+// Resolves `scratch` on device dev, which must be current; once per device.
+void noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    g_noalloc_err[dev] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    return;
+  }
+  g_noalloc_ptr[dev] = p;
+  g_noalloc_err[dev] = hipSuccess;
+}
+
+void* noalloc_ptr(int dev, hipError_t* err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  *err = g_noalloc_err[dev];
+  return g_noalloc_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -323,6 +364,7 @@ int tc_ctx_create(int device, tc_ctx** out) {
   }
   e = hipSetDevice(dev);
   if (e != hipSuccess) return TC_EHIP;
+  noalloc_setup(dev);  // tc_count_noalloc_i32's array on this device
   auto* c = new tc_ctx;
   c->device = dev;
   int cus = 0;
@@ -433,6 +475,27 @@ int tc_count_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const
   int rc = ensure(c, L.total);
   if (rc) return rc;
   return count32(c, c->pool, L, cfg, vi, nv, nnz, d_off, d_ind, d_out, d_total, static_cast<hipStream_t>(stream));
+}
+
+int tc_count_noalloc_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                         int32_t* d_out, int64_t* d_total, const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (nv < 0 || nnz < 0 || !d_off || (nnz > 0 && !d_ind) || (nv > 0 && !d_out))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  if (!cfg.oriented) return set_err(c, TC_EINVAL, "the no-allocation count runs the oriented scheme only");
+  const Layout L = layout(nv, nnz, false, Scheme::kOrient);
+  if (L.total > TC_NOALLOC_SCRATCH_BYTES)
+    return set_err(c, TC_ERANGE, "graph needs " + std::to_string(L.total) + " bytes of scratch, more than the " +
+                                     std::to_string(TC_NOALLOC_SCRATCH_BYTES) + "-byte no-allocation array");
+  hipError_t e = hipSuccess;
+  void* base = noalloc_ptr(c->device, &e);
+  if (!base) return hip_fail(c, e, "no-allocation scratch address");
+  if ((e = hipSetDevice(c->device)) != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  return count32(c, base, L, cfg, vi, nv, nnz, d_off, d_ind, d_out, d_total, static_cast<hipStream_t>(stream));
 }
 
 namespace {

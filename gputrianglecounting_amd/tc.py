@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
         "tc_ctx_set_timing": (C.c_int, [vp, C.c_int]),
         "tc_last_timing": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "tc_count_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
+        "tc_count_noalloc_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_kernelCall_i32": (C.c_int, [C.c_uint, C.c_uint, i32, vp, vp, vp, i32, i32, i32]),
         "tc_count_i64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_count_total_i64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp]),
@@ -135,7 +136,7 @@ def lib() -> C.CDLL:
 # Every symbol include/tc.h declares (checked by the CPU test suite).
 EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes", "tc_ctx_reserve",
            "tc_ctx_set_default_cfg", "tc_ctx_get_default_cfg", "tc_ctx_set_timing", "tc_last_timing",
-           "tc_count_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_count_total_i64", "tc_lcc_f64",
+           "tc_count_i32", "tc_count_noalloc_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_count_total_i64", "tc_lcc_f64",
            "tc_validate_i32",
            "tc_batch_count_i32",
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
@@ -259,6 +260,17 @@ class TriangleCounter:
         rc = lib().tc_count_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, _p(out) if nv else None,
                                 _p(total), _cfgp(cfg), _stream(stream))
         self._check(rc, "tc_count_i32")
+        return out, total
+
+    def count_noalloc(self, off, ind, out, total=None, cfg=None, stream=None):
+        """:meth:`count` through ``tc_count_noalloc_i32``: the library's fixed
+        per-GPU scratch array, no device allocation.  ``out`` (and ``total``)
+        must be given; graphs beyond the array raise TCError (TC_ERANGE)."""
+        nv = int(off.shape[0]) - 1
+        nnz = int(ind.shape[0])
+        rc = lib().tc_count_noalloc_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None,
+                                        _p(out) if nv else None, _p(total), _cfgp(cfg), _stream(stream))
+        self._check(rc, "tc_count_noalloc_i32")
         return out, total
 
     def count64(self, off, ind, out=None, total=None, cfg=None, stream=None):

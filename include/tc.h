@@ -135,6 +135,25 @@ int tc_last_timing(tc_ctx* ctx, float ms[4]);
 int tc_count_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
                  int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, void* stream);
 
+/* Per-vertex count that allocates nothing: tc_count_i32 on the oriented
+ * scheme (the default), with its scratch in a fixed device array of
+ * TC_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on
+ * until the process ends (its address is resolved once per device by
+ * tc_ctx_create).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating library routine.  Same arguments, output and
+ * asynchrony as tc_count_i32; TC_ERANGE when the graph's scratch
+ * (about 28 (nv + 1) + 4 nnz + 0.3 MB bytes) exceeds the array, TC_EINVAL
+ * when cfg asks for a scheme other than the oriented one.  Every call on
+ * one GPU shares that GPU's array and leaves it holding its own working
+ * data (each call clears what must start at zero): calls that overlap on
+ * one GPU, from different streams or host threads, race and must be
+ * ordered by the caller. */
+#define TC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tc_count_noalloc_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                         int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, void* stream);
+
 /* Shape-compatible stand-in for kernelCall<int32_t>
  * (src/clusteringCount.cu:415-467): same argument list and meaning.
  * numberThreads must be one of the reference block sizes
