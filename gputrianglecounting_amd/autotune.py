@@ -14,7 +14,7 @@ import os
 import time
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
-CFG_KEYS = ("wg_size", "items", "ring", "small_max", "ranges_per_wg", "algo", "chunk", "depth", "search_ratio", "probe_ratio", "oriented", "row_groups")
+CFG_KEYS = ("wg_size", "items", "ring", "small_max", "ranges_per_wg", "algo", "chunk", "depth", "search_ratio", "probe_ratio", "oriented", "row_groups", "lookup", "long_rows")
 
 
 def load_table(path: str = TABLE) -> dict:

@@ -62,6 +62,8 @@ tc_cfg builtin_default() {
   c.probe_ratio = 4;
   c.oriented = 3;     // automatic (orient_side)
   c.row_groups = -1;  // automatic (orient_groups)
+  c.lookup = 0;       // automatic (orient_lookup)
+  c.long_rows = 0;    // automatic (orient_long)
   return c;
 }
 
@@ -81,6 +83,8 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->probe_ratio) c.probe_ratio = in->probe_ratio;
     if (in->oriented) c.oriented = in->oriented > 0 ? (in->oriented > 3 ? 3 : in->oriented) : 0;
     if (in->row_groups) c.row_groups = in->row_groups;
+    if (in->lookup) c.lookup = in->lookup;
+    if (in->long_rows) c.long_rows = in->long_rows;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -95,6 +99,7 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
   if (c.row_groups > 0 && c.row_groups != 1 && c.row_groups != 2 && c.row_groups != 4 && c.row_groups != 8 &&
       c.row_groups != 16 && c.row_groups != 32)
     return -1;
+  if (c.lookup < 0 || c.lookup > 3 || c.long_rows < 0 || c.long_rows > 2) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   if (c.algo != 0) c.oriented = 0;  // the oriented kernel is algo 0's
@@ -109,9 +114,12 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 // 4: 222), s20 8, the mixed 4096-graph batch 16, RGG n=4M 16-32.  A batch
 // is sized by its mean degree only: its lists are those of its (small)
 // graphs, whatever the batch's total.
+int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch);
 int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false) {
   if (cfg.row_groups > 0) return cfg.row_groups;
   if (!batch && nnz >= (int64_t(1) << 30)) return 1;
+  // the table kernel streams one partner per wave (s24: 175.0 ms; 2: 199.8)
+  if (!batch && nnz >= (int64_t(1) << 28) && orient_lookup(cfg, nnz, batch) == 1) return 1;
   if (!batch && nnz >= (int64_t(1) << 28)) return 2;
   const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
   if (dbar < 10) return 32;
@@ -143,8 +151,35 @@ int orient_side(const tc_cfg& cfg, bool batch, int64_t nnz) {
   return batch || nnz < kSideArcs ? 1 : 2;
 }
 
+// How the oriented count meets a partner's list with the row's (the
+// launcher's lookup argument): 0 = the row window as a sorted LDS array
+// walked by each lane (k_count_orient), 1 = an LDS cuckoo table probed
+// twice per element of wave-coalesced reads (k_count_hash), 2 = the same
+// kernel searching a sorted window per element.  cfg.lookup 1, 2 and 3 ask
+// for them; 0 (automatic) takes the table for single graphs from 2^28 arcs
+// up and the walked array otherwise.  Measured count times, walked/table:
+// R-MAT s24 188.7/175.0 ms (both with long rows), s20 6.3/8.1 ms, RGG n=4M
+// 1.7/3.3 ms: the table's coalesced reads pay off on long partner lists,
+// its build on short rows does not.
+constexpr int64_t kTableArcs = int64_t(1) << 28;
+int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch) {
+  if (cfg.lookup >= 1 && cfg.lookup <= 3) return cfg.lookup - 1;
+  return !batch && nnz >= kTableArcs ? 1 : 0;
+}
+// Rows longer than a wave's LDS window counted by workgroups sharing one
+// table (the table kernel's long-row phase, or k_count_long after the walked
+// array) rather than in windows: cfg.long_rows 2 = always, 1 = never,
+// 0 (automatic) = from 2^26 arcs up.  Measured at R-MAT s24: walked array
+// 209.8 -> 188.7 ms, table 202.5 -> 175.0 ms; at s20 and RGG n=4M (few long
+// rows) the two extra launches cost 1-3%.
+constexpr int64_t kLongArcs = int64_t(1) << 26;
+bool orient_long(const tc_cfg& cfg, int64_t nnz) {
+  return cfg.long_rows == 2 || (cfg.long_rows == 0 && nnz >= kLongArcs);
+}
+
 struct Layout {
-  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, odeg, vmark, vcnt, total;
+  size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, odeg, lrow, vmark, vcnt,
+      total;
   int64_t ecap;
   Scheme sc;
 };
@@ -177,6 +212,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.prow = take(orient ? 4 * (size_t)(nnz / tcd::kOrientPiece + 2 + tcd::kOrientPieceSlack) : 0);
   L.huge = take(orient ? 4 * (size_t)(nnz / tcd::kHugeRow + 1) : 0);
   L.odeg = take(orient ? 4 * (size_t)(nv + 1) : 0);
+  L.lrow = take(orient ? 4 * (size_t)(nnz / 512 + 2) : 0);
   L.vmark = take(list ? 4 * (size_t)(nv + 1) : 0);
   L.vcnt = take(counts ? 4 * (size_t)(nv + 1) : 0);
   L.total = p;
@@ -230,6 +266,7 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
   p.prow = orient ? reinterpret_cast<int32_t*>(b + L.prow) : nullptr;
   p.huge = orient ? reinterpret_cast<int32_t*>(b + L.huge) : nullptr;
   p.odeg = orient ? reinterpret_cast<int32_t*>(b + L.odeg) : nullptr;
+  p.lrow = orient ? reinterpret_cast<int32_t*>(b + L.lrow) : nullptr;
   p.vmark = list ? reinterpret_cast<int32_t*>(b + L.vmark) : nullptr;
   p.vcnt = counts ? reinterpret_cast<int32_t*>(b + L.vcnt) : nullptr;
   return p;
@@ -297,7 +334,8 @@ int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, i
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
     if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
-                                      s)) != hipSuccess)
+                                      s, tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
+        hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
@@ -523,7 +561,8 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
   if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
-                                      s)) != hipSuccess)
+                                      s, tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
+        hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -574,7 +613,9 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   record(c, 1, s);
   const int groups = orient_groups(cfg, nv, nnz);
   if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups,
-                                    orient_side(cfg, false, nnz), c->cus, s)) != hipSuccess)
+                                    orient_side(cfg, false, nnz), c->cus, s, tcd::kOrientPiece,
+                                    orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
+        hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -613,7 +654,8 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
     const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, orient_side(cfg, false, nnz), c->cus, s)) !=
+    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, orient_side(cfg, false, nnz), c->cus, s,
+                                      tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
         hipSuccess)
       return hip_fail(c, e, "count kernel launch");
     c->last_dscal = p.dscal;
@@ -681,7 +723,9 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     const int groups = orient_groups(cfg, nv_total, nnz_total, true);
     const int side = orient_side(cfg, true, nnz_total);
     if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, side, c->cus, s,
-                                      tcd::kOrientPieceBatch)) != hipSuccess)
+                                      tcd::kOrientPieceBatch, orient_lookup(cfg, nnz_total, true),
+                                      orient_long(cfg, nnz_total))) !=
+        hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
@@ -858,7 +902,8 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
   (void)tcd::resident_wgs2(vi, c->cus, 1);
   const Scheme sc = scheme_of(cfg);
-  if (sc == Scheme::kOrient) (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz, batch), 1, c->cus);
+  if (sc == Scheme::kOrient)
+    (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz, batch), 1, c->cus, orient_lookup(cfg, nnz, batch));
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 

@@ -94,6 +94,8 @@ struct Prep {
   // rows of more than kHugeRow arcs, oriented one per wave before the rest
   int32_t* huge;    // [nnz / kHugeRow + 1]
   int32_t* odeg;    // [nv + 1]: every vertex's degree, read by the orientation
+  // rows whose N+ exceeds a wave's LDS window, counted by k_count_long
+  int32_t* lrow;    // [nnz / 512 + 2]
 };
 
 size_t scan_tmp_elems(int64_t n_cap);
@@ -117,8 +119,9 @@ int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
 int resident_wgs2(int variant_idx, int cus, int which);
 // Resident workgroups of the oriented count kernel; kind 0 = the total
-// only, 1 = int32 counts, 2 = int64 counts.
-int resident_orient_wgs(int groups, int kind, int cus);
+// only, 1 = int32 counts, 2 = int64 counts; lookup 0 = the sorted-window
+// merge (k_count_orient), else the table lookups (k_count_hash).
+int resident_orient_wgs(int groups, int kind, int cus, int lookup = 0);
 // nwg, nwg2, nwg3: grids of the streamed kernel's wave launch, its
 // workgroup launch and the lookup kernel (persistent: resident workgroups).
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
@@ -131,11 +134,20 @@ hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int3
 // higher vertex of each arc (partners are its in-neighbours); 0: the lower
 // one (partners are its out-neighbours); 2: 1 unless the orientation of the
 // same call listed rows of more than kHugeRow arcs (p.dscal[6]), else 0.
-// piece: the base piece size (orient_piece picks the one used).
+// piece: the base piece size (orient_piece picks the one used).  lookup 0:
+// the row window is a sorted LDS array that every lane walks
+// (k_count_orient); 1: an LDS cuckoo table probed twice per element, the
+// partner lists read in wave-coalesced 16-byte vectors (k_count_hash);
+// 2: the same kernel with every window a sorted array searched per element.
+// lng: rows whose N+ is longer than the per-wave kernel's LDS window are
+// listed (k_long_rows) and counted by workgroups sharing one table of up
+// to 2496 ids (k_count_long) instead of in windows by single waves.
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece);
+                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece, int lookup = 0,
+                               bool lng = false);
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece);
+                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece, int lookup = 0,
+                               bool lng = false);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

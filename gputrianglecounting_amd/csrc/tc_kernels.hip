@@ -718,6 +718,7 @@ __device__ __forceinline__ int32_t stream_chunks_a(const int32_t* __restrict__ b
 
 #include "tc_bs_kernel.h"
 #include "tc_orient_kernel.h"
+#include "tc_hash_kernel.h"
 #include "tc_search_kernel.h"
 
 // One lane per edge, two-pointer merge from global memory.  Simple and
@@ -983,7 +984,7 @@ constexpr bool kOrientPF = true;
 constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
-                          unsigned long long*, unsigned long long*, int, int32_t);
+                          unsigned long long*, unsigned long long*, int, int32_t, int64_t);
 // partners per wave (groups), per-vertex output type (or the total only),
 // one piece per wave -> kernel
 template <bool PV, typename OT, bool ONE = false>
@@ -997,55 +998,152 @@ OrientFn<OT> orient_fn(int groups) {
     default: return k_count_orient<kOrientAW, 4, kOrientPF, kOrientWPE, PV, OT, ONE>;
   }
 }
-// kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts
-const void* orient_kernel(int groups, int kind) {
-  if (kind == 1) return reinterpret_cast<const void*>(orient_fn<true, int32_t>(groups));
-  if (kind == 2) return reinterpret_cast<const void*>(orient_fn<true, int64_t>(groups));
-  return reinterpret_cast<const void*>(orient_fn<false, int32_t>(groups));
+// Table slots per wave and window of the hash-lookup kernel: 24 TS bytes of
+// LDS per workgroup (keys and 16-bit counters of four waves); a window of AW
+// ids fills a table of 2 * roundup32(2 AW / 3) <= TS slots to at most 3/4.
+constexpr int kHashTS = 832;
+constexpr int kHashAW = 624;
+constexpr int kHashWPE = 8;
+// Rows with more ids in N+ than this go to the table kernel's long-row
+// phase (at most kHashAW, the per-wave window).
+constexpr int kHashLong = kHashAW;
+template <typename OT>
+using HashFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
+                        unsigned long long*, unsigned long long*, int, int32_t, int, int64_t, const int32_t*,
+                        const unsigned long long*, int64_t, unsigned long long*);
+// WIN: rows longer than the window are walked in windows (without it the
+// kernel counts them first, in its long-row phase)
+template <bool PV, typename OT, bool ONE = false, bool WIN = true>
+HashFn<OT> hash_fn(int groups) {
+  switch (groups) {
+    case 1: return k_count_hash<kHashTS, kHashAW, 1, kHashWPE, PV, OT, ONE, WIN>;
+    case 2: return k_count_hash<kHashTS, kHashAW, 2, kHashWPE, PV, OT, ONE, WIN>;
+    case 8: return k_count_hash<kHashTS, kHashAW, 8, kHashWPE, PV, OT, ONE, WIN>;
+    case 16: return k_count_hash<kHashTS, kHashAW, 16, kHashWPE, PV, OT, ONE, WIN>;
+    case 32: return k_count_hash<kHashTS, kHashAW, 32, kHashWPE, PV, OT, ONE, WIN>;
+    default: return k_count_hash<kHashTS, kHashAW, 4, kHashWPE, PV, OT, ONE, WIN>;
+  }
 }
-int resident_orient_wgs(int groups, int kind, int cus) {
+// Table slots of the long-row kernel: one table over the workgroup's LDS
+// (6 TSW bytes), rows of up to 3 TSW / 4 = 2496 ids in one window.
+constexpr int kLongTS = 3328;
+template <typename OT>
+using LongFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, const unsigned long long*,
+                        int64_t, OT*, unsigned long long*, const unsigned long long*, unsigned long long*, int);
+// partners per wave at once: 1, 2 or 4 (groups above 4 take 4: long rows'
+// partner lists are long)
+template <bool PV, typename OT>
+LongFn<OT> long_fn(int groups) {
+  if (groups <= 1) return k_count_long<kLongTS, 1, PV, OT>;
+  if (groups == 2) return k_count_long<kLongTS, 2, PV, OT>;
+  return k_count_long<kLongTS, 4, PV, OT>;
+}
+int resident_long_wgs(int groups, int kind, int cus) {
   static std::mutex mu;
-  static int occ[3][33] = {};
-  const int gi = groups == 1 || groups == 2 || groups == 8 || groups == 16 || groups == 32 ? groups : 4;
+  static int occ[3][3] = {};
+  const int gi = groups <= 1 ? 0 : groups == 2 ? 1 : 2;
   const int ki = kind == 1 || kind == 2 ? kind : 0;
   std::lock_guard<std::mutex> lk(mu);
   if (!occ[ki][gi]) {
+    const void* f = ki == 1   ? reinterpret_cast<const void*>(long_fn<true, int32_t>(groups))
+                    : ki == 2 ? reinterpret_cast<const void*>(long_fn<true, int64_t>(groups))
+                              : reinterpret_cast<const void*>(long_fn<false, int32_t>(groups));
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, orient_kernel(gi, ki), 256, 0) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess || n < 1) n = 1;
     occ[ki][gi] = n;
   }
   return occ[ki][gi] * cus;
 }
+// kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts; lookup 0 =
+// k_count_orient, else k_count_hash
+const void* orient_kernel(int groups, int kind, int lookup) {
+  if (lookup) {
+    if (kind == 1) return reinterpret_cast<const void*>(hash_fn<true, int32_t>(groups));
+    if (kind == 2) return reinterpret_cast<const void*>(hash_fn<true, int64_t>(groups));
+    return reinterpret_cast<const void*>(hash_fn<false, int32_t>(groups));
+  }
+  if (kind == 1) return reinterpret_cast<const void*>(orient_fn<true, int32_t>(groups));
+  if (kind == 2) return reinterpret_cast<const void*>(orient_fn<true, int64_t>(groups));
+  return reinterpret_cast<const void*>(orient_fn<false, int32_t>(groups));
+}
+int resident_orient_wgs(int groups, int kind, int cus, int lookup) {
+  static std::mutex mu;
+  static int occ[2][3][33] = {};
+  const int gi = groups == 1 || groups == 2 || groups == 8 || groups == 16 || groups == 32 ? groups : 4;
+  const int ki = kind == 1 || kind == 2 ? kind : 0;
+  const int li = lookup ? 1 : 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!occ[li][ki][gi]) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, orient_kernel(gi, ki, li), 256, 0) != hipSuccess || n < 1)
+      n = 1;
+    occ[li][ki][gi] = n;
+  }
+  return occ[li][ki][gi] * cus;
+}
 
 template <bool PV, typename OT>
 hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total, int groups,
-                                 int side, int kind, int cus, hipStream_t s, int64_t base) {
+                                 int side, int kind, int cus, hipStream_t s, int64_t base, int lookup, bool lng) {
   if (base < kOrientPieceMin || 2 * base >= 65536 || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
-  const int nwg = resident_orient_wgs(groups, kind, cus);
+  const int nwg = resident_orient_wgs(groups, kind, cus, lookup);
   const int64_t piece = orient_piece(nnz, base, 4 * (int64_t)nwg);
   hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, piece,
                      p.prow);
   // a wave for every piece: the kernel that takes one piece per wave
-  const OrientFn<OT> fn = 4 * (int64_t)nwg * piece >= nnz ? orient_fn<PV, OT, true>(groups) : orient_fn<PV, OT>(groups);
-  hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
-                     (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total),
-                     ctr, side, (int32_t)piece);
+  const bool one = 4 * (int64_t)nwg * piece >= nnz;
+  // rows longer than a wave's window are listed first; the table kernel
+  // counts them in its first phase, the walked-array kernel leaves them to
+  // k_count_long
+  const int64_t aw = lookup ? kHashLong : kOrientAW;
+  const int64_t lmax = lng ? aw : INT64_MAX;
+  unsigned long long* lq = reinterpret_cast<unsigned long long*>(p.qctr);  // [0] rows listed, [1] rows taken
+  const int64_t lcap = nnz / aw + 1;
+  if (lng) {
+    if (!p.lrow) return hipErrorInvalidValue;
+    if ((e = launch_fill_i64(reinterpret_cast<int64_t*>(p.qctr), 2, 0, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_long_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.orow, nv, aw,
+                       p.lrow, lcap, lq);
+  }
+  if (lookup) {
+    const HashFn<OT> fn = lng ? (one ? hash_fn<PV, OT, true, false>(groups) : hash_fn<PV, OT, false, false>(groups))
+                              : (one ? hash_fn<PV, OT, true>(groups) : hash_fn<PV, OT>(groups));
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
+                       (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out,
+                       reinterpret_cast<unsigned long long*>(total), ctr, side, (int32_t)piece, lookup == 2 ? 1 : 0,
+                       lmax, (const int32_t*)p.lrow, (const unsigned long long*)lq, lcap, lq + 1);
+    return hipGetLastError();  // the long rows were counted in its first phase
+  } else {
+    const OrientFn<OT> fn = one ? orient_fn<PV, OT, true>(groups) : orient_fn<PV, OT>(groups);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
+                       (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out,
+                       reinterpret_cast<unsigned long long*>(total), ctr, side, (int32_t)piece, lmax);
+  }
+  if (lng) {
+    const LongFn<OT> lf = long_fn<PV, OT>(groups);
+    hipLaunchKernelGGL(lf, dim3(resident_long_wgs(groups, kind, cus)), dim3(256), 0, s, (const int64_t*)p.voff,
+                       (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.lrow,
+                       (const unsigned long long*)lq, lcap, out, reinterpret_cast<unsigned long long*>(total),
+                       (const unsigned long long*)ctr, lq + 1, side);
+  }
   return hipGetLastError();
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece) {
-  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 1, cus, s, piece)
-             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, side, 0, cus, s, piece);
+                               int side, int cus, hipStream_t s, int64_t piece, int lookup, bool lng) {
+  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 1, cus, s, piece, lookup, lng)
+             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, side, 0, cus, s, piece, lookup, lng);
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece) {
-  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, side, 0, cus, s, piece);
-  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 2, cus, s, piece);
+                               int side, int cus, hipStream_t s, int64_t piece, int lookup, bool lng) {
+  if (!out)
+    return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, side, 0, cus, s, piece, lookup,
+                                        lng);
+  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 2, cus, s, piece, lookup, lng);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {

@@ -121,7 +121,7 @@ template <int AW, int G, bool PF, int WPE, bool PV, typename OT, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
-    unsigned long long* ctr, int side, int32_t P) {
+    unsigned long long* ctr, int side, int32_t P, int64_t lmax) {
   constexpr int NW = 4;
   constexpr int CH = 8;
   constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       const int64_t p0 = by_in ? oe : t_lo, p1 = by_in ? t_next : oe;
       const int64_t tu = p0 > a0 ? p0 : a0;
       const int64_t lt = (p1 < a1 ? p1 : a1) - tu;  // partners of row u in this piece
-      if (lu == 0 || lt <= 0) {
+      if (lu == 0 || lt <= 0 || lu > lmax) {  // rows over lmax are k_count_long's
         if (t_next >= a1) break;
         continue;
       }

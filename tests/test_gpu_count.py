@@ -56,7 +56,12 @@ _ORIENTED = [dict(oriented=1), dict(oriented=1, probe_ratio=1), dict(oriented=1,
              dict(oriented=1, row_groups=8, probe_ratio=1),
              dict(oriented=2), dict(oriented=2, row_groups=1), dict(oriented=2, row_groups=8, probe_ratio=1),
              dict(oriented=1, row_groups=4), dict(oriented=1, row_groups=16), dict(oriented=1, row_groups=32),
-             dict(oriented=2, row_groups=32)]
+             dict(oriented=2, row_groups=32),
+             dict(lookup=2), dict(lookup=2, oriented=1), dict(lookup=2, oriented=2), dict(lookup=2, row_groups=1),
+             dict(lookup=2, oriented=1, row_groups=2), dict(lookup=2, oriented=1, row_groups=8),
+             dict(lookup=2, oriented=2, row_groups=32), dict(lookup=1), dict(lookup=3), dict(lookup=3, row_groups=1),
+             dict(lookup=3, oriented=2, row_groups=16), dict(lookup=2, long_rows=2),
+             dict(long_rows=2, row_groups=1), dict(lookup=3, long_rows=2, oriented=2, row_groups=8)]
 
 
 @pytest.mark.parametrize("cfg", _ORIENTED, ids=str)
@@ -64,6 +69,38 @@ def test_oriented(counter, cfg):
     """Degree-oriented per-vertex counting (k_count_orient): each triangle
     found once and its three vertices credited; same counts as the oracle."""
     for g in GR.small_cases() + GR.random_cases() + _search_graphs():
+        _check(counter, g, cfg=cfg)
+
+
+@pytest.mark.parametrize("cfg", [dict(long_rows=2), dict(long_rows=2, row_groups=1), dict(long_rows=2, oriented=2),
+                                 dict(long_rows=2, lookup=2), dict(long_rows=2, lookup=2, oriented=2, row_groups=8),
+                                 dict(long_rows=2, lookup=3), dict(long_rows=1, lookup=2), dict(long_rows=1)], ids=str)
+def test_long_rows(counter, cfg):
+    """Rows whose N+ exceeds a wave's LDS window, counted by whole
+    workgroups sharing one table (k_count_long) or in windows: dense
+    graphs (K_1800, rows of up to 1799 out-neighbours), hub-heavy R-MAT, and
+    two hubs adjacent to everything (in-neighbour partners beyond the
+    16-bit counter run of 16384)."""
+    import os
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = 1800
+    g = GR.complete(n)
+    out, tot = counter.count_graph(g, cfg=cfg)
+    assert (out == (n - 1) * (n - 2)).all() and tot == n * (n - 1) * (n - 2)
+    k = 40000
+    leaves = np.arange(2, k + 2)
+    g2 = G.from_edges(k + 2, np.concatenate([[0], np.zeros(k, np.int64), np.ones(k, np.int64)]),
+                      np.concatenate([[1], leaves, leaves]), name=f"k2_join_{k}")
+    exp = np.full(k + 2, 2, np.int64)
+    exp[:2] = 2 * k
+    out, tot = counter.count_graph(g2, cfg=cfg)
+    np.testing.assert_array_equal(out, exp)
+    assert tot == 6 * k
+    for g in [G.rmat(17, 16, seed=8), G.rmat(14, 64, seed=9), G.geometric(600000, 8.0, seed=3)]:
+        key = (g.name, g.n, g.nnz)
+        if key not in _REF:
+            _REF[key] = oracle.per_vertex(g.off, g.ind, threads=threads)
         _check(counter, g, cfg=cfg)
 
 
