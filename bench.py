@@ -3,18 +3,24 @@
 Default workload: R-MAT scale 24, edge factor 16 (a=.57 b=.19 c=.19),
 seed 1, simple symmetric int32 CSR (~520M arcs), counted by the default
 configuration: the degree orientation written in place of the input rows
-(k_huge_rows, k_orient_rows), then k_count_orient.  A step is one full
+(k_huge_rows, k_orient_rows), then k_count_hash (a cuckoo table of each
+row in LDS, partner lists read in wave-coalesced vectors; rows longer than
+a wave's table counted first by whole workgroups).  A step is one full
 tc_count_i32 call over the resident graph: the orientation, the count
 kernel and the int64 total.  The metric is undirected edges / step time
 (edge-intersections/s); the roofline entry is for the count kernel, timed
 with HIP events on its launch stream.  The default run adds the batched
-comparison: 4096 mixed graphs counted as one batch against one launch
-sequence per graph.
+comparison (4096 mixed graphs counted as one batch against one launch
+sequence per graph, with the batched kernel's own roofline entry) and a
+short serving leg (a burst, then Poisson arrivals at half the burst rate:
+graphs/s and p50/p99 latency).
 
 The CPU baseline runs the reference's own intersection (oracle/_ref, or the
 C restatement when that is absent) on a bounded uniform sample of arcs of
 the same graph on one host core; its full-graph time is extrapolated by arc
-count.
+count.  Beside it, the OpenMP restatement of the vertex-list count runs on
+all the host threads the run may use (at most 16) over a uniform vertex
+sample, extrapolated the same way.
 """
 from __future__ import annotations
 
@@ -105,14 +111,16 @@ def algorithmic_bytes(g, small_max: int, search_ratio: int) -> dict:
                 stream_units=stream_units)
 
 
-def algorithmic_bytes_oriented(g) -> dict:
+def algorithmic_bytes_oriented(g, by_in: bool = False) -> dict:
     """Bytes of the oriented per-vertex count by definition.
 
-    Arcs v -> w are kept iff (d(v), v) < (d(w), w).  Each oriented row u is
-    read once as A (4 d+(u) bytes), and for every arc u -> w the row N+(w)
-    is streamed (4 d+(w) bytes) after its two row offsets (16 bytes); counts
-    and the total are written once.  Reading the CSR to orient it is prep,
-    outside the count kernel."""
+    Arcs v -> w are kept iff (d(v), v) < (d(w), w).  Each oriented row is
+    read once into the table (4 d+ bytes), and for every arc u -> w one
+    partner list is streamed after its two row offsets (16 bytes): N+(w)
+    when rows take their out-neighbours as partners (the default for large
+    single graphs), N+(u) when they take their in-neighbours (``by_in``,
+    batches and smaller graphs).  Counts and the total are written once.
+    Reading the CSR to orient it is prep, outside the count kernel."""
     off = g.off.astype(np.int64)
     ind = g.ind
     d = np.diff(off)
@@ -135,7 +143,7 @@ def algorithmic_bytes_oriented(g) -> dict:
         src = np.repeat(np.arange(r0, r1, dtype=np.int64), d[r0:r1])
         dst = ind[off[r0]:off[r1]].astype(np.int64)
         keep = (d[src] < d[dst]) | ((d[src] == d[dst]) & (src < dst))
-        stream += int(dplus[dst[keep]].sum())
+        stream += int(dplus[(src if by_in else dst)[keep]].sum())
     b = 4.0 * (arcs + stream) + 16.0 * arcs + 4 * g.n + 8
     return dict(b_kernel=b, oriented_arcs=arcs, oriented_stream_elements=stream, max_out_degree=int(dplus.max()))
 
@@ -172,6 +180,57 @@ def cpu_baseline(g, target_s: float) -> dict:
             "sample": (f"{k2} arcs drawn uniformly (seed 12345) from the same graph, visited in CSR order by "
                        f"{'the reference count_intersectionsReg (clusteringCount.cpp:2-48)' if use_ref else 'the C restatement'}"
                        f" on 1 core ({cpu}); {t:.1f} s, full-graph time extrapolated by arc count: {full_s:.0f} s")}
+
+
+class _CSR:
+    """A plain global CSR (the batch as one disjoint graph) for the byte count."""
+
+    def __init__(self, off, ind):
+        self.off, self.ind = off, ind
+        self.n, self.nnz = int(off.shape[0] - 1), int(ind.shape[0])
+
+
+def batch_as_graph(b) -> _CSR:
+    """The packed batch as one disjoint CSR in global vertex ids: graph g's
+    rows are v_base[g] .. v_base[g+1], its offsets rebased by arc_base[g]
+    (the same map k_voff_batch applies on the device)."""
+    vb, ab = b.tab[0::2].astype(np.int64), b.tab[1::2].astype(np.int64)
+    G_ = b.count
+    ng = np.diff(vb)
+    gid = np.repeat(np.arange(G_, dtype=np.int64), ng)
+    off = b.off[np.arange(vb[G_], dtype=np.int64) + gid].astype(np.int64) + ab[:G_][gid]
+    off = np.append(off, ab[G_])
+    del gid
+    ind = b.ind.astype(np.int64) + np.repeat(vb[:G_], np.diff(ab))
+    return _CSR(off, ind.astype(np.int32))
+
+
+def cpu_baseline_all_cores(g, target_s: float) -> dict:
+    """The OpenMP restatement of the reference's vertex-list count on all
+    the host threads this run may use (at most 16), over a uniform sample of
+    vertices: every arc is in the sample with the same probability, so the
+    full-graph time is extrapolated by arc count."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(54321)
+    d = g.degrees()
+
+    def run(k: int):
+        vl = np.sort(rng.choice(g.n, size=k, replace=False)).astype(np.int32)
+        t0 = time.perf_counter()
+        oracle.vertex_list_omp(vl, g.off, g.ind, threads=threads)
+        return time.perf_counter() - t0, int(d[vl].sum())
+
+    k = 2000
+    t, a = run(k)
+    k2 = int(min(g.n, max(k, k * target_s / max(t, 1e-3))))
+    t, a = run(k2)
+    full_s = t * g.nnz / max(a, 1)
+    return {"value": (g.nnz // 2) / full_s, "unit": "edge-intersections/s", "cores": threads, "kind": "port",
+            "sample": f"{k2} vertices drawn uniformly (seed 54321), {a} arcs, counted by the OpenMP restatement of "
+                      f"clusteringCount.cpp:125-155 on {threads} threads; {t:.1f} s, full-graph time extrapolated "
+                      f"by arc count: {full_s:.0f} s"}
 
 
 def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
@@ -223,12 +282,28 @@ def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
     deg = dv[keep]
     inv = bool((h % 2 == 0).all() and (h <= deg * (deg - 1)).all())
     graph.close()
+    # roofline of the batched count kernel: one eager count with the
+    # context's HIP events around it (on the launch stream), against the
+    # oriented bytes of the batch as one disjoint graph (in-neighbour
+    # partners, the batches' side)
+    tc.set_timing(True)
+    ks = []
+    for _ in range(3):
+        tc.batch_count(tab, off, ind, b.total_vertices, b.total_arcs, out, totals)
+        ks.append(tc.last_timing().count_ms)
+    tc.set_timing(False)
+    k_ms = float(np.median(ks))
+    ab = algorithmic_bytes_oriented(batch_as_graph(b), by_in=True)
+    roof = {"bound": "hbm", "achieved": round(ab["b_kernel"] / (k_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ab["b_kernel"] / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_count_orient (batch, in-neighbour partners)", "count_kernel_ms": round(k_ms, 3),
+            "b_kernel_bytes": ab["b_kernel"]}
     tc.set_default_cfg(**saved)
     return {"graphs": b.count, "vertices": b.total_vertices, "arcs": b.total_arcs,
             "batched_graphs_per_s": b.count / batched_s, "batched_ms": batched_s * 1e3,
             "per_graph_graphs_per_s": b.count / per_s, "per_graph_ms": per_s * 1e3,
             "speedup": per_s / batched_s, "identical_to_per_graph": same, "invariants": inv,
-            "kernel_cfg": cfg, "autotuned": tuned, "captured": "hipGraph"}
+            "kernel_cfg": cfg, "autotuned": tuned, "captured": "hipGraph", "roofline": roof}
 
 
 def run_serve(tc, pool: int = 48, per_rate: int = 192, rel_rates=(0.25, 0.5, 0.75, 0.9),
@@ -338,6 +413,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cfg", default="", help="kernel config, e.g. wg_size=256,items=8,ring=4096")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-graph comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the short serving leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the count runs on a single MI355X (no multi-GPU path is built); "
@@ -395,7 +471,6 @@ def main() -> int:
         tc.count(off, ind, out, total, cfg=cfg)
         torch.cuda.synchronize()
         log(f"warmup step {tc.last_timing()}")
-    E, W = tc.last_edge_work()
 
     # full-size checks (untimed): invariants and an independent total from
     # the degree-ordered global-count mode
@@ -442,6 +517,9 @@ def main() -> int:
     value = edges / (ms_per_step / 1e3)
     kcfg = tc.default_cfg() if cfg is None else dict(tc.default_cfg(), **cfg)
     oriented = kcfg["algo"] == 0 and kcfg["oriented"] > 0
+    # edges intersected and their merge work exist only on the edge scheme
+    # (the oriented count builds no edge list)
+    edge_work = {} if oriented else dict(zip(("intersected_edges", "merge_work"), tc.last_edge_work()))
     ab = algorithmic_bytes(g, kcfg["small_max"], kcfg["search_ratio"] if kcfg["algo"] in (0, 3) else -1)
     if oriented:
         ab.update(algorithmic_bytes_oriented(g))
@@ -450,11 +528,13 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(config["workload"])}
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cpu = cpu_baseline(g, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.3g} {cpu['unit']}")
+        cpu_all = cpu_baseline_all_cores(g, args.cpu_seconds)
+        log(f"cpu baseline, {cpu_all['cores']} threads: {cpu_all['value']:.3g} {cpu_all['unit']}")
 
     batch = None
     if not args.no_batch and args.workload == "rmat24":
@@ -466,16 +546,27 @@ def main() -> int:
         except Exception as exc:  # the headline line must still print
             batch = {"error": repr(exc)}
 
+    serving = None
+    if not args.no_serve and args.workload == "rmat24":
+        log("serving: burst and one Poisson rate")
+        try:
+            serving = run_serve(tc, pool=24, per_rate=64, rel_rates=(0.5,))
+            log(f"serving {serving['saturation_graphs_per_s']:.1f} graphs/s, at 50%: p50 "
+                f"{serving['rates'][0]['p50_ms']} ms p99 {serving['rates'][0]['p99_ms']} ms")
+        except Exception as exc:  # the headline line must still print
+            serving = {"error": repr(exc)}
+
     line = {
         "metric": METRIC, "value": value, "unit": "edge-intersections/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int64" if args.int64 else "int32",
         "data": "synthetic: seeded R-MAT/ER/RGG from the package's counter-based generators",
-        "config": dict(config, n=g.n, arcs=g.nnz, undirected_edges=edges, intersected_edges=E, merge_work=W,
+        "config": dict(config, n=g.n, arcs=g.nnz, undirected_edges=edges, **edge_work,
                        csr="int64" if args.int64 else "int32",
                        kernel_cfg=tc.default_cfg() if cfg is None else cfg),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "count_kernel_ms": round(float(np.mean(kern)), 3),
         "prep_ms": round(float(np.mean(prep)), 3),
         "count_scheme": "oriented rows" if oriented else "undirected edges",
@@ -483,13 +574,14 @@ def main() -> int:
         "b_alg_all_edges_bytes": ab["b_alg_all"],
         "hbm_gbs_all_edges_formula": round(ab["b_alg_all"] / t_k / 1e9, 1),
         "batched": batch,
+        "serving": serving,
         "oriented": {k: ab[k] for k in ("oriented_arcs", "oriented_stream_elements", "max_out_degree")} if oriented
         else None,
         "checks": {"invariants": inv_ok, "total": tot, "global_mode_total": tot_global,
                    "int64_csr_per_vertex_identical": int64_identical,
                    "global_mode_count_ms": round(global_ms, 3), "edge_scheme_total": tot_edges,
                    "edge_scheme_per_vertex_identical": edges_identical,
-                   "host_edge_count_agrees": True if oriented else ab["edges"] == E,
+                   "host_edge_count_agrees": True if oriented else ab["edges"] == edge_work["intersected_edges"],
                    "totals_agree": tot == tot_global == tot_edges, "triangles": tot // 6},
     }
     print(json.dumps(line), flush=True)

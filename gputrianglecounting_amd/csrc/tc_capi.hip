@@ -169,13 +169,21 @@ int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch) {
 // Rows longer than a wave's LDS window counted by workgroups sharing one
 // table (the table kernel's long-row phase, or k_count_long after the walked
 // array) rather than in windows: cfg.long_rows 2 = always, 1 = never,
-// 0 (automatic) = from 2^26 arcs up.  Measured at R-MAT s24: walked array
-// 209.8 -> 188.7 ms, table 202.5 -> 175.0 ms; at s20 and RGG n=4M (few long
-// rows) the two extra launches cost 1-3%.
+// 0 (automatic) = single graphs from 2^26 arcs up.  Measured at R-MAT s24:
+// walked array 209.8 -> 188.7 ms, table 202.5 -> 175.0 ms; at s20 and RGG
+// n=4M (few long rows) the two extra launches cost 1-3%, and on the mixed
+// 4096-graph batch (small graphs, hardly any long row) 33.2 -> 33.7 ms.
 constexpr int64_t kLongArcs = int64_t(1) << 26;
-bool orient_long(const tc_cfg& cfg, int64_t nnz) {
-  return cfg.long_rows == 2 || (cfg.long_rows == 0 && nnz >= kLongArcs);
+bool orient_long(const tc_cfg& cfg, int64_t nnz, bool batch) {
+  return cfg.long_rows == 2 || (cfg.long_rows == 0 && !batch && nnz >= kLongArcs);
 }
+
+// The oriented count kernels of one call over the orientation in p, with
+// cfg's automatic choices resolved for this size (partners per wave, partner
+// side, lookup, long rows) and the piece size of single graphs or batches.
+template <typename OT>
+hipError_t count_oriented(const tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int64_t nv, int64_t nnz, OT* out,
+                          int64_t* total, hipStream_t s, bool batch = false);
 
 struct Layout {
   size_t voff, vbase, ecnt, eoff, eu, ew, ework, wpre, dscal, qctr, scan, orow, oind, prow, huge, odeg, lrow, vmark, vcnt,
@@ -272,6 +280,15 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
   return p;
 }
 
+template <typename OT>
+hipError_t count_oriented(const tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int64_t nv, int64_t nnz, OT* out,
+                          int64_t* total, hipStream_t s, bool batch) {
+  return tcd::launch_count_orient(p, nv, nnz, out, total, orient_groups(cfg, nv, nnz, batch),
+                                  orient_side(cfg, batch, nnz), c->cus, s,
+                                  batch ? tcd::kOrientPieceBatch : tcd::kOrientPiece, orient_lookup(cfg, nnz, batch),
+                                  orient_long(cfg, nnz, batch));
+}
+
 void record(tc_ctx* c, int i, hipStream_t s) {
   if (c->timing && c->ev[i]) (void)hipEventRecord(c->ev[i], s);
 }
@@ -332,10 +349,7 @@ int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, i
     if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
     if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
-    const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
-                                      s, tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
-        hipSuccess)
+    if ((e = count_oriented(c, p, cfg, nv, nnz, d_out, d_total, s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
@@ -559,10 +573,7 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
-  const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, nnz, d_out, d_total, groups, orient_side(cfg, false, nnz), c->cus,
-                                      s, tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
-        hipSuccess)
+  if ((e = count_oriented(c, p, cfg, nv, nnz, d_out, d_total, s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -611,11 +622,7 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
   if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
   record(c, 1, s);
-  const int groups = orient_groups(cfg, nv, nnz);
-  if ((e = tcd::launch_count_orient(p, nv, nnz, static_cast<int32_t*>(nullptr), d_total, groups,
-                                    orient_side(cfg, false, nnz), c->cus, s, tcd::kOrientPiece,
-                                    orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
-        hipSuccess)
+  if ((e = count_oriented(c, p, cfg, nv, nnz, static_cast<int32_t*>(nullptr), d_total, s)) != hipSuccess)
     return hip_fail(c, e, "count kernel launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
@@ -653,10 +660,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
     if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
     record(c, 1, s);
-    const int groups = orient_groups(cfg, nv, nnz);
-    if ((e = tcd::launch_count_orient(p, nv, nnz, p.vcnt, nullptr, groups, orient_side(cfg, false, nnz), c->cus, s,
-                                      tcd::kOrientPiece, orient_lookup(cfg, nnz, false), orient_long(cfg, nnz))) !=
-        hipSuccess)
+    if ((e = count_oriented(c, p, cfg, nv, nnz, p.vcnt, nullptr, s)) != hipSuccess)
       return hip_fail(c, e, "count kernel launch");
     c->last_dscal = p.dscal;
     record(c, 2, s);
@@ -720,11 +724,7 @@ int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_to
     if ((e = tcd::launch_orient(p, nv_total, nnz_total, d_ind_cat, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "orient");
     record(c, 1, s);
-    const int groups = orient_groups(cfg, nv_total, nnz_total, true);
-    const int side = orient_side(cfg, true, nnz_total);
-    if ((e = tcd::launch_count_orient(p, nv_total, nnz_total, d_out_cat, nullptr, groups, side, c->cus, s,
-                                      tcd::kOrientPieceBatch, orient_lookup(cfg, nnz_total, true),
-                                      orient_long(cfg, nnz_total))) !=
+    if ((e = count_oriented(c, p, cfg, nv_total, nnz_total, d_out_cat, static_cast<int64_t*>(nullptr), s, true)) !=
         hipSuccess)
       return hip_fail(c, e, "count kernel launch");
   } else {

@@ -97,7 +97,7 @@ typedef struct tc_ctx tc_ctx;
  *                  3) are counted by whole workgroups sharing one table of
  *                  up to 2496 ids (k_count_long) instead of in windows by
  *                  single waves; 1 = windows; 0 (default) = automatic: 2
- *                  from 2^26 arcs up, else 1 */
+ *                  for single graphs from 2^26 arcs up, else 1 */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
