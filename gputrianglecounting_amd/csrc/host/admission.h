@@ -6,7 +6,9 @@
 // A batch of G requests occupies ONE arena block laid out as the batched
 // counter's packed CSR (tab rows {v_base, arc_base}, concatenated offsets
 // and ids, then the per-vertex counts and per-graph totals), so the block
-// can be handed to the batched entry point as it is.
+// can be handed to the batched entry point as it is, followed by the
+// scratch of the batch's count: the arena bounds all the device memory of
+// the batches in flight.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -78,15 +80,28 @@ class Arena {
   std::map<size_t, size_t> free_;  // offset -> length
 };
 
+// Device scratch of a batched count of nv vertices and nnz arcs on the
+// oriented scheme (tc_capi.hip's layout): 32 bytes per vertex (widened
+// offsets, graph bases, N+ bounds, degrees), 4 per arc (the orientation)
+// and fewer than 1/32 per arc more (piece, hub-row and long-row lists), and
+// a fixed part (the piece list's slack of 65,536 entries, counters, the
+// rounding of ten regions) within kScratchFixed.
+constexpr size_t kScratchFixed = 1040 * kArenaAlign;
+inline size_t scratch_share(int64_t nv, int64_t nnz) {
+  return 32 * (size_t)nv + 4 * (size_t)nnz + ((size_t)nnz + 31) / 32;
+}
+inline size_t scratch_bound(int64_t nv, int64_t nnz) { return kScratchFixed + scratch_share(nv, nnz); }
+
 // Byte offsets of the regions of one batch block, each kArenaAlign aligned:
 //   tab   int64[2(G+1)]     rows {v_base, arc_base}, row G = totals
 //   off   int32[nv + G]     graph g's offsets at v_base[g] + g
 //   ind   int32[nnz]        graph g's ids at arc_base[g]
 //   out   int32[max(nv,1)]  per-vertex counts
 //   tot   int64[G]          per-graph totals
-// [0, upload_end) goes host -> device, [out, bytes) device -> host.
+//   scr   scratch_bound(nv, nnz) bytes of the count's scratch
+// [0, upload_end) goes host -> device, [out, res_end) device -> host.
 struct BatchLayout {
-  size_t tab, off, ind, out, tot, upload_end, bytes;
+  size_t tab, off, ind, out, tot, res_end, scr, upload_end, bytes;
 };
 inline BatchLayout batch_layout(int64_t G, int64_t nv, int64_t nnz) {
   BatchLayout L;
@@ -102,6 +117,9 @@ inline BatchLayout batch_layout(int64_t G, int64_t nv, int64_t nnz) {
   p += arena_round(4 * (size_t)(nv > 0 ? nv : 1));
   L.tot = p;
   p += arena_round(8 * (size_t)G);
+  L.res_end = p;
+  L.scr = p;
+  p += arena_round(scratch_bound(nv, nnz));
   L.bytes = p;
   return L;
 }
@@ -179,13 +197,17 @@ inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, 
 }
 
 // Batch block size as a sum over requests: each request contributes its
-// exact share (a tab row, nv+1 offsets, nnz ids, nv counts, one total) and
-// the block adds kBatchFixed, which covers the tab's closing row and the
-// rounding of the five regions (each < kArenaAlign).  So
-// batch_layout(G, sum nv, sum nnz).bytes <= batch_bytes(sum of shares).
-constexpr size_t kBatchFixed = 6 * kArenaAlign;
-inline size_t request_share(int64_t nv, int64_t nnz) { return 28 + 8 * (size_t)nv + 4 * (size_t)nnz; }
-inline size_t batch_bytes(size_t share_sum) { return arena_round(kBatchFixed + share_sum); }
+// share (a tab row, nv+1 offsets, nnz ids, nv counts, one total, and its
+// scratch_share) and the block adds kBatchFixed, which covers the tab's
+// closing row and the rounding of the six regions (each < kArenaAlign), and
+// kScratchFixed.  So batch_layout(G, sum nv, sum nnz).bytes <=
+// batch_bytes(sum of shares).
+constexpr size_t kBatchFixed = 7 * kArenaAlign;
+constexpr size_t kBlockFixed = kBatchFixed + kScratchFixed;
+inline size_t request_share(int64_t nv, int64_t nnz) {
+  return 28 + 8 * (size_t)nv + 4 * (size_t)nnz + scratch_share(nv, nnz);
+}
+inline size_t batch_bytes(size_t share_sum, size_t fixed = kBlockFixed) { return arena_round(fixed + share_sum); }
 // Arena bytes of a request batched alone.
 inline size_t request_bytes(int64_t nv, int64_t nnz) { return batch_bytes(request_share(nv, nnz)); }
 
@@ -200,10 +222,21 @@ struct Pending {
 
 class Admission {
  public:
-  explicit Admission(size_t budget) : arena_(budget) {}
+  // fixed: the bytes a block adds to its requests' shares (kBlockFixed for
+  // the serving loop's blocks)
+  explicit Admission(size_t budget, size_t fixed = kBlockFixed) : arena_(budget), fixed_(fixed) {}
   Arena& arena() { return arena_; }
   // False when the request can never fit (larger than the whole budget).
-  bool fits_ever(size_t share) const { return batch_bytes(share) <= arena_.capacity(); }
+  bool fits_ever(size_t share) const { return batch_bytes(share, fixed_) <= arena_.capacity(); }
+  // Takes request id out of the queue; false when it is not waiting.
+  bool remove(int64_t id) {
+    for (auto it = q_.begin(); it != q_.end(); ++it)
+      if (it->id == id) {
+        q_.erase(it);
+        return true;
+      }
+    return false;
+  }
   void push(int64_t id, size_t share) { q_.push_back(Pending{id, share}); }
   size_t waiting() const { return q_.size(); }
   // Admit one batch: fills ids and returns the block offset and size, or
@@ -215,12 +248,12 @@ class Admission {
     size_t sum = 0;
     int n = 0;
     for (const Pending& p : q_) {
-      if (n >= max_n || batch_bytes(sum + p.share) > room) break;
+      if (n >= max_n || batch_bytes(sum + p.share, fixed_) > room) break;
       sum += p.share;
       ++n;
     }
     if (n == 0) return SIZE_MAX;
-    bytes = batch_bytes(sum);
+    bytes = batch_bytes(sum, fixed_);
     const size_t off = arena_.alloc(bytes);
     if (off == SIZE_MAX) return SIZE_MAX;  // not reached: room >= bytes
     for (int i = 0; i < n; ++i) {
@@ -232,6 +265,7 @@ class Admission {
 
  private:
   Arena arena_;
+  size_t fixed_;
   std::deque<Pending> q_;
 };
 

@@ -10,6 +10,7 @@
 // (csrdatacreate.cpp:45); tch_from_edges/tch_normalize produce the simple
 // symmetric CSR the per-vertex count is defined on.
 #include "tc_host.h"
+#include "csr_check.h"
 
 #include <omp.h>
 
@@ -380,6 +381,14 @@ tch_graph* tch_gen_rgg(int32_t n, double mean_deg, uint64_t seed) {
   return graph_from_arcs(n, s, d);
 }
 
+namespace {
+// TCH_E* code of a row fault (tcs::row_fault).
+int row_code(int f) {
+  return f == tcs::kRowRange ? TCH_ERANGE : f == tcs::kRowUnsorted ? TCH_EUNSORTED
+       : f == tcs::kRowSelfLoop ? TCH_ESELFLOOP : f == tcs::kRowAsym ? TCH_EASYM : 0;
+}
+}  // namespace
+
 int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* ind) {
   if (n < 0 || nnz < 0 || !off || (nnz > 0 && !ind)) return fail(TCH_EINVAL, "invalid argument");
   if (nnz > INT32_MAX) return fail(TCH_ERANGE, "nnz exceeds int32");
@@ -390,19 +399,7 @@ int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* 
   std::atomic<int64_t> where{INT64_MAX};
 #pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t v = 0; v < n; ++v) {
-    int c = 0;
-    for (int32_t k = off[v]; k < off[v + 1] && !c; ++k) {
-      int32_t w = ind[k];
-      if (w < 0 || w >= n) c = TCH_ERANGE;
-      else if (k > off[v] && ind[k - 1] >= w) c = TCH_EUNSORTED;
-      else if (w == v) c = TCH_ESELFLOOP;
-    }
-    if (!c) {
-      for (int32_t k = off[v]; k < off[v + 1] && !c; ++k) {
-        int32_t w = ind[k];
-        if (!std::binary_search(ind + off[w], ind + off[w + 1], (int32_t)v)) c = TCH_EASYM;
-      }
-    }
+    const int c = row_code(tcs::row_fault(n, off, ind, v));
     if (c) {
       int64_t prev = where.load();
       while (v < prev && !where.compare_exchange_weak(prev, v)) {
@@ -414,14 +411,7 @@ int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* 
   if (code.load()) {
     // Re-derive the code of the lowest offending vertex for a stable message.
     int64_t v = where.load();
-    int c = 0;
-    for (int32_t k = off[v]; k < off[v + 1] && !c; ++k) {
-      int32_t w = ind[k];
-      if (w < 0 || w >= n) c = TCH_ERANGE;
-      else if (k > off[v] && ind[k - 1] >= w) c = TCH_EUNSORTED;
-      else if (w == v) c = TCH_ESELFLOOP;
-      else if (!std::binary_search(ind + off[w], ind + off[w + 1], (int32_t)v)) c = TCH_EASYM;
-    }
+    const int c = row_code(tcs::row_fault(n, off, ind, v));
     const char* what = c == TCH_ERANGE ? "id out of range" : c == TCH_EUNSORTED ? "row not strictly increasing"
                        : c == TCH_ESELFLOOP ? "self-loop" : "arc without reverse";
     return fail(c, std::string(what) + " at vertex " + std::to_string(v));
@@ -752,7 +742,7 @@ int tch_admission_sim(int64_t budget, int32_t max_batch, int32_t nsteps, const i
                       int32_t* admitted_at, int32_t* batch_of, int64_t* peak_used) {
   if (budget <= 0 || max_batch < 1 || nsteps < 0 || (nsteps > 0 && (!script || !admitted_at || !batch_of)))
     return fail(TCH_EINVAL, "invalid argument");
-  tcs::Admission adm((size_t)budget);
+  tcs::Admission adm((size_t)budget, tcs::kBatchFixed);  // the policy alone: shares are given in bytes
   struct Blk {
     size_t off, bytes;
     int32_t live;

@@ -365,8 +365,78 @@ int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, i
   return TC_OK;
 }
 
-std::mutex g_shim_mu;
-tc_ctx* g_shim_ctx[64] = {nullptr};
+// The batched count (tc_batch_count_i32's launches) with its scratch in
+// the context's pool, or at base (base_bytes long, 256-byte aligned; the
+// serving loop's arena blocks) when base is given.  Allocates nothing in
+// the second case.
+int batch_count(tc_ctx* c, void* base, size_t base_bytes, int32_t G, const int64_t* d_tab, int64_t nv_total,
+                int64_t nnz_total, const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat,
+                int64_t* d_totals, const tc_cfg* cfg_in, void* stream) {
+  if (!c) return TC_EINVAL;
+  if (G <= 0 || !d_tab || !d_off_cat || nv_total < 0 || nnz_total < 0 || (nnz_total > 0 && !d_ind_cat) ||
+      (nv_total > 0 && !d_out_cat))
+    return set_err(c, TC_EINVAL, "invalid argument");
+  if (nv_total >= INT32_MAX) return set_err(c, TC_ERANGE, "total vertex count beyond int32");
+  tc_cfg cfg;
+  const int vi = resolve(c, cfg_in, &cfg);
+  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+  const Layout L = layout(nv_total, nnz_total, true, scheme_of(cfg));
+  int rc = TC_OK;
+  if (base) {
+    if (L.total > base_bytes) return set_err(c, TC_ENOMEM, "batch scratch beyond the given region");
+  } else if ((rc = ensure(c, L.total)) != TC_OK) {
+    return rc;
+  }
+  tcd::Prep p = make_prep(base ? base : c->pool, L, true);
+  record(c, 0, s);
+  if ((e = tcd::launch_fill_i32(d_out_cat, nv_total, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+  if ((e = tcd::launch_voff_batch(d_tab, G, d_off_cat, nv_total, p.voff, p.vbase, s)) != hipSuccess)
+    return hip_fail(c, e, "voff");
+  if (L.sc == Scheme::kOrient) {
+    // The batch is one disjoint graph in global ids: the orientation
+    // (graph-local ids translated through p.vbase) never joins two graphs,
+    // so every triangle stays inside its graph.
+    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_orient(p, nv_total, nnz_total, d_ind_cat, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "orient");
+    record(c, 1, s);
+    if ((e = count_oriented(c, p, cfg, nv_total, nnz_total, d_out_cat, static_cast<int64_t*>(nullptr), s, true)) !=
+        hipSuccess)
+      return hip_fail(c, e, "count kernel launch");
+  } else {
+    if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
+      return hip_fail(c, e, "edge list");
+    record(c, 1, s);
+    rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
+    if (rc) return rc;
+  }
+  c->last_dscal = p.dscal;
+  record(c, 2, s);
+  if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
+    return hip_fail(c, e, "totals");
+  record(c, 3, s);
+  c->ev_recorded = c->timing;
+  return TC_OK;
+}
+
+// Contexts of the kernelCall stand-ins: one per device and calling thread,
+// so concurrent callers never share scratch; destroyed when the thread ends.
+struct ShimContexts {
+  tc_ctx* ctx[64] = {nullptr};
+  ~ShimContexts() {
+    for (tc_ctx*& c : ctx)
+      if (c) tc_ctx_destroy(c);
+  }
+};
+tc_ctx* shim_ctx(int dev) {
+  static thread_local ShimContexts held;
+  if (dev < 0 || dev >= 64) return nullptr;
+  if (!held.ctx[dev] && tc_ctx_create(dev, &held.ctx[dev]) != TC_OK) held.ctx[dev] = nullptr;
+  return held.ctx[dev];
+}
 
 // Device address of `scratch` per device (nullptr until resolved) and the
 // error of a failed lookup.
@@ -697,50 +767,8 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
 int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,
                        const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat, int64_t* d_totals,
                        const tc_cfg* cfg_in, void* stream) {
-  if (!c) return TC_EINVAL;
-  if (G <= 0 || !d_tab || !d_off_cat || nv_total < 0 || nnz_total < 0 || (nnz_total > 0 && !d_ind_cat) ||
-      (nv_total > 0 && !d_out_cat))
-    return set_err(c, TC_EINVAL, "invalid argument");
-  if (nv_total >= INT32_MAX) return set_err(c, TC_ERANGE, "total vertex count beyond int32");
-  tc_cfg cfg;
-  const int vi = resolve(c, cfg_in, &cfg);
-  if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = hipSetDevice(c->device);
-  if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-  const Layout L = layout(nv_total, nnz_total, true, scheme_of(cfg));
-  int rc = ensure(c, L.total);
-  if (rc) return rc;
-  tcd::Prep p = make_prep(c->pool, L, true);
-  record(c, 0, s);
-  if ((e = tcd::launch_fill_i32(d_out_cat, nv_total, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_voff_batch(d_tab, G, d_off_cat, nv_total, p.voff, p.vbase, s)) != hipSuccess)
-    return hip_fail(c, e, "voff");
-  if (L.sc == Scheme::kOrient) {
-    // The batch is one disjoint graph in global ids: the orientation
-    // (graph-local ids translated through p.vbase) never joins two graphs,
-    // so every triangle stays inside its graph.
-    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-    if ((e = tcd::launch_orient(p, nv_total, nnz_total, d_ind_cat, c->cus, s)) != hipSuccess)
-      return hip_fail(c, e, "orient");
-    record(c, 1, s);
-    if ((e = count_oriented(c, p, cfg, nv_total, nnz_total, d_out_cat, static_cast<int64_t*>(nullptr), s, true)) !=
-        hipSuccess)
-      return hip_fail(c, e, "count kernel launch");
-  } else {
-    if ((e = tcd::launch_build_edges(p, nv_total, p.voff, d_ind_cat, 0, c->cus, s)) != hipSuccess)
-      return hip_fail(c, e, "edge list");
-    record(c, 1, s);
-    rc = run_count(c, p, cfg, vi, d_ind_cat, d_out_cat, nullptr, 2, s);
-    if (rc) return rc;
-  }
-  c->last_dscal = p.dscal;
-  record(c, 2, s);
-  if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
-    return hip_fail(c, e, "totals");
-  record(c, 3, s);
-  c->ev_recorded = c->timing;
-  return TC_OK;
+  return batch_count(c, nullptr, 0, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, d_totals, cfg_in,
+                     stream);
 }
 
 int tc_lcc_f64(tc_ctx* c, int32_t nv, const int32_t* d_off, const int32_t* d_out, double* d_lcc, void* stream) {
@@ -791,14 +819,8 @@ int tc_kernelCall_i32(unsigned numberBlocks, unsigned numberThreads, int32_t nv,
     return TC_EINVAL;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TC_EHIP;
-  tc_ctx* c;
-  {
-    std::lock_guard<std::mutex> lk(g_shim_mu);
-    if (!g_shim_ctx[dev]) {
-      if (tc_ctx_create(dev, &g_shim_ctx[dev]) != TC_OK) return TC_EHIP;
-    }
-    c = g_shim_ctx[dev];
-  }
+  tc_ctx* c = shim_ctx(dev);
+  if (!c) return TC_EHIP;
   // nnz from the device row offsets, as the reference kernel reads them.
   int32_t nnz = 0;
   hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
@@ -819,14 +841,8 @@ int tc_kernelCall_i64(unsigned numberBlocks, unsigned numberThreads, int64_t nv,
     return TC_EINVAL;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TC_EHIP;
-  tc_ctx* c;
-  {
-    std::lock_guard<std::mutex> lk(g_shim_mu);
-    if (!g_shim_ctx[dev]) {
-      if (tc_ctx_create(dev, &g_shim_ctx[dev]) != TC_OK) return TC_EHIP;
-    }
-    c = g_shim_ctx[dev];
-  }
+  tc_ctx* c = shim_ctx(dev);
+  if (!c) return TC_EHIP;
   int64_t nnz = 0;
   hipError_t e = hipMemcpy(&nnz, d_off + nv, sizeof nnz, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return TC_EHIP;
@@ -841,6 +857,10 @@ struct tc_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int device = 0;
+  // The context the call was recorded on: its scratch is the graph's own
+  // (allocated before recording, freed with the graph), so later counts on
+  // the caller's context, or other graphs, never move or share it.
+  tc_ctx* own = nullptr;
 };
 
 namespace {
@@ -886,7 +906,17 @@ int capture(tc_ctx* c, F&& fn, tc_graph** out) {
   r->graph = g;
   r->exec = x;
   r->device = c->device;
+  r->own = c;
   *out = r;
+  return TC_OK;
+}
+
+// A context for one recorded call: the caller's device, defaults and
+// timing flag, and scratch of its own.
+int private_ctx(const tc_ctx* c, tc_ctx** out) {
+  int rc = tc_ctx_create(c->device, out);
+  if (rc != TC_OK) return rc;
+  (*out)->def = c->def;
   return TC_OK;
 }
 
@@ -1017,8 +1047,12 @@ int autotune(tc_ctx* c, Run&& run, int32_t* d_out, int64_t n, int reps, int set_
   if (n_written) *n_written = written;
   if (rc) return rc;
   if (best_i >= 0) {
-    if (best) *best = cands[best_i];
-    if (set_default) c->def = cands[best_i];
+    // the winner as it ran: zero fields resolved against the default in
+    // force while tuning, so using it later changes nothing the timing saw
+    tc_cfg r{};
+    if (resolve(c, &cands[best_i], &r) < 0) return set_err(c, TC_EINVAL, "tuned configuration does not resolve");
+    if (best) *best = r;
+    if (set_default) c->def = r;
   }
   if (any_diff) return set_err(c, TC_EGRAPH, "a tuned configuration produced different counts");
   return TC_OK;
@@ -1055,10 +1089,18 @@ int tc_count_capture_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_of
                          int32_t* d_out, int64_t* d_total, const tc_cfg* cfg, tc_graph** out) {
   if (!c || !out) return TC_EINVAL;
   *out = nullptr;
-  int rc = prepare_capture(c, cfg, nv, nnz, false);
-  if (rc) return rc;
-  return capture(
-      c, [&](hipStream_t s) { return tc_count_i32(c, nv, nnz, d_off, d_ind, d_out, d_total, cfg, s); }, out);
+  tc_ctx* own = nullptr;
+  int rc = private_ctx(c, &own);
+  if (rc) return set_err(c, rc, "cannot create the graph's context");
+  rc = prepare_capture(own, cfg, nv, nnz, false);
+  if (!rc)
+    rc = capture(
+        own, [&](hipStream_t s) { return tc_count_i32(own, nv, nnz, d_off, d_ind, d_out, d_total, cfg, s); }, out);
+  if (rc) {
+    set_err(c, rc, own->err);
+    tc_ctx_destroy(own);
+  }
+  return rc;
 }
 
 int tc_batch_capture_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,
@@ -1066,15 +1108,23 @@ int tc_batch_capture_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_
                          const tc_cfg* cfg, tc_graph** out) {
   if (!c || !out) return TC_EINVAL;
   *out = nullptr;
-  int rc = prepare_capture(c, cfg, nv_total, nnz_total, true);
-  if (rc) return rc;
-  return capture(
-      c,
-      [&](hipStream_t s) {
-        return tc_batch_count_i32(c, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, d_totals, cfg,
-                                  s);
-      },
-      out);
+  tc_ctx* own = nullptr;
+  int rc = private_ctx(c, &own);
+  if (rc) return set_err(c, rc, "cannot create the graph's context");
+  rc = prepare_capture(own, cfg, nv_total, nnz_total, true);
+  if (!rc)
+    rc = capture(
+        own,
+        [&](hipStream_t s) {
+          return tc_batch_count_i32(own, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, d_totals, cfg,
+                                    s);
+        },
+        out);
+  if (rc) {
+    set_err(c, rc, own->err);
+    tc_ctx_destroy(own);
+  }
+  return rc;
 }
 
 int tc_graph_launch(tc_graph* g, void* stream) {
@@ -1088,6 +1138,7 @@ int tc_graph_destroy(tc_graph* g) {
   if (!g) return TC_EINVAL;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->own) tc_ctx_destroy(g->own);
   delete g;
   return TC_OK;
 }
@@ -1113,3 +1164,19 @@ int tc_autotune_batch_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ serving loop
+// Internal interface of tc_server.hip (same library; not exported).
+namespace tci {
+// Scratch bytes of a batched count of nv vertices and nnz arcs on the
+// oriented scheme (the serving loop's contexts keep the default scheme).
+size_t batch_orient_scratch(int64_t nv, int64_t nnz) { return layout(nv, nnz, true, Scheme::kOrient).total; }
+// tc_batch_count_i32 with the scratch at base (base_bytes long).
+int batch_count_at(tc_ctx* c, void* base, size_t base_bytes, int32_t G, const int64_t* d_tab, int64_t nv_total,
+                   int64_t nnz_total, const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat,
+                   int64_t* d_totals, void* stream) {
+  if (!base) return TC_EINVAL;
+  return batch_count(c, base, base_bytes, G, d_tab, nv_total, nnz_total, d_off_cat, d_ind_cat, d_out_cat, d_totals,
+                     nullptr, stream);
+}
+}  // namespace tci

@@ -9,10 +9,16 @@
 //           hbm_budget bytes
 //   launch  on a free slot: the batch is packed into the arena block's
 //           pinned host mirror in the batched counter's layout, uploaded with
-//           one copy on the slot's stream, counted by tc_batch_count_i32 and
-//           its counts copied back with one copy, then an event is recorded
+//           one copy on the slot's stream, counted by the batched counter
+//           with its scratch in the rest of the same block and its counts
+//           copied back with one copy, then an event is recorded
 // so requests that arrive while batches run are admitted as blocks free up
-// and batches on different slots overlap on the device.
+// and batches on different slots overlap on the device.  Every byte of
+// device memory a batch uses is in its block, so hbm_budget bounds them all,
+// and nothing is allocated on the device after tc_server_create.
+// Requests are checked on the host when submitted (a simple undirected CSR:
+// ids in range, rows strictly increasing, no self-loops, every arc with its
+// reverse), so one malformed request can never touch another's block.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -25,7 +31,15 @@
 #include <vector>
 
 #include "host/admission.h"
+#include "host/csr_check.h"
 #include "tc.h"
+
+// tc_capi.hip: the batched count with the scratch at a given address.
+namespace tci {
+int batch_count_at(tc_ctx* c, void* base, size_t base_bytes, int32_t G, const int64_t* d_tab, int64_t nv_total,
+                   int64_t nnz_total, const int32_t* d_off_cat, const int32_t* d_ind_cat, int32_t* d_out_cat,
+                   int64_t* d_totals, void* stream);
+}
 
 namespace {
 
@@ -111,17 +125,18 @@ int tc_server::launch(Slot& sl) {
   }
   hipStream_t s = sl.stream;
   if (nnz_total == 0) {  // no arcs: every count and total is zero
-    std::memset(h + sl.L.out, 0, sl.L.bytes - sl.L.out);
+    std::memset(h + sl.L.out, 0, sl.L.res_end - sl.L.out);
     return hipEventRecord(sl.done, s) == hipSuccess ? TC_OK : TC_EHIP;
   }
   if (hipMemcpyAsync(d, h, sl.L.upload_end, hipMemcpyHostToDevice, s) != hipSuccess) return TC_EHIP;
-  const int rc = tc_batch_count_i32(sl.ctx, G, reinterpret_cast<const int64_t*>(d + sl.L.tab), nv_total, nnz_total,
-                                    reinterpret_cast<const int32_t*>(d + sl.L.off),
-                                    reinterpret_cast<const int32_t*>(d + sl.L.ind),
-                                    reinterpret_cast<int32_t*>(d + sl.L.out), reinterpret_cast<int64_t*>(d + sl.L.tot),
-                                    nullptr, s);
+  const int rc = tci::batch_count_at(sl.ctx, d + sl.L.scr, sl.L.bytes - sl.L.scr, G,
+                                     reinterpret_cast<const int64_t*>(d + sl.L.tab), nv_total, nnz_total,
+                                     reinterpret_cast<const int32_t*>(d + sl.L.off),
+                                     reinterpret_cast<const int32_t*>(d + sl.L.ind),
+                                     reinterpret_cast<int32_t*>(d + sl.L.out),
+                                     reinterpret_cast<int64_t*>(d + sl.L.tot), s);
   if (rc != TC_OK) return rc;
-  if (hipMemcpyAsync(h + sl.L.out, d + sl.L.out, sl.L.bytes - sl.L.out, hipMemcpyDeviceToHost, s) != hipSuccess)
+  if (hipMemcpyAsync(h + sl.L.out, d + sl.L.out, sl.L.res_end - sl.L.out, hipMemcpyDeviceToHost, s) != hipSuccess)
     return TC_EHIP;
   if (hipEventRecord(sl.done, s) != hipSuccess) return TC_EHIP;
   return TC_OK;
@@ -270,6 +285,11 @@ int tc_server_destroy(tc_server* s) {
 int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h_off, const int32_t* h_ind) {
   if (!s || nv < 0 || nnz < 0 || nnz > INT32_MAX || !h_off || (nnz > 0 && !h_ind)) return TC_EINVAL;
   if (h_off[0] != 0 || h_off[nv] != nnz) return TC_EINVAL;
+  if (tcs::csr_fault(nv, nnz, h_off, h_ind)) {  // checked before anything is queued or launched
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.rejected++;
+    return TC_EGRAPH;
+  }
   const size_t share = tcs::request_share(nv, nnz);
   auto r = std::make_unique<Request>();
   r->nv = nv;
@@ -314,6 +334,24 @@ int tc_server_wait(tc_server* s, int64_t id, int32_t* h_out, int64_t* h_total, i
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   s->reqs.erase(it);
   return err;
+}
+
+int tc_server_cancel(tc_server* s, int64_t id) {
+  if (!s) return TC_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return TC_EINVAL;
+  Request* r = it->second.get();
+  if (r->done || !s->adm->remove(id)) return 1;  // admitted already: it completes as usual
+  r->done = true;
+  r->err = TC_ECANCELED;
+  r->t_done = clk::now();
+  std::vector<int32_t>().swap(r->off);
+  std::vector<int32_t>().swap(r->ind);
+  s->st.canceled++;
+  s->st.waiting = (int64_t)s->adm->waiting();
+  s->cv_done.notify_all();
+  return TC_OK;
 }
 
 int tc_server_get_stats(tc_server* s, tc_server_stats* st) {

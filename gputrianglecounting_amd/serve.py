@@ -24,7 +24,7 @@ class ServerCfg(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "completed", "rejected", "batches", "bytes_in_use",
-                                         "peak_bytes", "waiting", "in_flight")]
+                                         "peak_bytes", "waiting", "in_flight", "canceled")]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -47,6 +47,8 @@ def _bind():
     L.tc_server_submit.argtypes = [vp, C.c_int32, C.c_int64, _I32P, _I32P]
     L.tc_server_wait.restype = C.c_int
     L.tc_server_wait.argtypes = [vp, C.c_int64, _I32P, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_double)]
+    L.tc_server_cancel.restype = C.c_int
+    L.tc_server_cancel.argtypes = [vp, C.c_int64]
     L.tc_server_get_stats.restype = C.c_int
     L.tc_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     _bound = True
@@ -96,6 +98,17 @@ class Server:
         self._nv[rid] = g.n
         return int(rid)
 
+    def cancel(self, rid: int) -> bool:
+        """Cancels a request still waiting for admission (True); False when it
+        was admitted already and completes as usual."""
+        rc = _bind().tc_server_cancel(self._h, rid)
+        if rc < 0:
+            raise T.TCError(f"tc_server_cancel({rid}) failed", rc)
+        if rc == T.TC_OK:
+            del self._nv[rid]
+            _bind().tc_server_wait(self._h, rid, None, None, 0, None)  # collect the canceled request
+        return rc == T.TC_OK
+
     def wait(self, rid: int, timeout_ms: int = -1) -> Result | None:
         n = self._nv[rid]
         out = np.empty(max(n, 1), np.int32)
@@ -144,13 +157,20 @@ def request_bytes(nv: int, nnz: int) -> int:
     return int(L.tch_request_bytes(nv, nnz))
 
 
+def scratch_share(nv: int, nnz: int) -> int:
+    """A request's share of its batch's count scratch (csrc/host/admission.h)."""
+    return 32 * nv + 4 * nnz + (nnz + 31) // 32
+
+
 def request_share(nv: int, nnz: int) -> int:
-    """A request's exact share of a batch block (one table row, nv+1
-    offsets, nnz ids, nv counts, one total)."""
-    return 28 + 8 * nv + 4 * nnz
+    """A request's share of a batch block: one table row, nv+1 offsets, nnz
+    ids, nv counts, one total, and its share of the count's scratch."""
+    return 28 + 8 * nv + 4 * nnz + scratch_share(nv, nnz)
 
 
-BATCH_FIXED = 6 * 256
+BATCH_FIXED = 7 * 256  # a block's table row and region rounding (the admission policy's fixed part)
+SCRATCH_FIXED = 1040 * 256  # the scratch's fixed part (piece-list slack, counters, rounding)
+BLOCK_FIXED = BATCH_FIXED + SCRATCH_FIXED  # what a serving block adds to its requests' shares
 
 
 def batch_block_bytes(G: int, nv_total: int, nnz_total: int) -> int:

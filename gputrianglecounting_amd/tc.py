@@ -20,7 +20,7 @@ import numpy as np
 
 from .graph import LIB_DIR, Batch, Graph
 
-TC_OK, TC_EINVAL, TC_ERANGE, TC_EHIP, TC_ENOMEM, TC_EGRAPH = 0, -1, -2, -3, -4, -5
+TC_OK, TC_EINVAL, TC_ERANGE, TC_EHIP, TC_ENOMEM, TC_EGRAPH, TC_ECANCELED = 0, -1, -2, -3, -4, -5, -6
 
 _lib = None
 
@@ -142,7 +142,7 @@ EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
            "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
-           "tc_server_submit", "tc_server_wait", "tc_server_get_stats"]
+           "tc_server_submit", "tc_server_wait", "tc_server_cancel", "tc_server_get_stats"]
 
 
 def _torch():

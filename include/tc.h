@@ -121,7 +121,8 @@ enum {
   TC_ERANGE = -2,    /* sizes beyond the int32 path (nnz >= 2^31) */
   TC_EHIP = -3,      /* a HIP runtime call failed (tc_last_error has the text) */
   TC_ENOMEM = -4,    /* device scratch could not be allocated */
-  TC_EGRAPH = -5     /* tc_validate_i32: input is not a simple undirected CSR */
+  TC_EGRAPH = -5,    /* tc_validate_i32, tc_server_submit: input is not a simple undirected CSR */
+  TC_ECANCELED = -6  /* tc_server_wait: the request was canceled (tc_server_cancel) */
 };
 
 /* Context: owns device scratch (edge list, work prefix, scan partials), the
@@ -317,15 +318,15 @@ int tc_copy_i32(tc_ctx* ctx, int32_t* d_dst, const int32_t* d_src, int64_t n, vo
  * longest prefix of the queue (at most max_batch_graphs requests) whose
  * block fits the largest free block of a device arena of hbm_budget bytes.
  * The block holds the batch in tc_batch_count_i32's packed layout (table,
- * offsets, ids) followed by its counts and totals, and stays allocated
- * until the batch completes, so hbm_budget bounds the device memory of the
- * requests in flight; each slot's context holds its own scratch on top.
- * A batch is packed into a pinned host mirror of its block, uploaded with
- * one copy, counted and copied back on one of `slots` launch slots (each
- * with its own stream and context), so batches overlap on the device while
- * later requests wait for memory.  Counts are those of tc_count_i32 on
- * each graph alone.  No reference counterpart (the reference counts one
- * graph per process, gpurun.cu:2-56). */
+ * offsets, ids), its counts and totals, and the scratch of its count, and
+ * stays allocated until the batch completes: hbm_budget bounds all the
+ * device memory of the requests in flight (the server allocates nothing on
+ * the device after tc_server_create).  A batch is packed into a pinned host
+ * mirror of its block, uploaded with one copy, counted and copied back on
+ * one of `slots` launch slots (each with its own stream), so batches overlap
+ * on the device while later requests wait for memory.  Counts are those of
+ * tc_count_i32 on each graph alone.  No reference counterpart (the
+ * reference counts one graph per process, gpurun.cu:2-56). */
 typedef struct tc_server tc_server;
 typedef struct tc_server_cfg {
   int32_t device;
@@ -337,19 +338,29 @@ typedef struct tc_server_cfg {
 typedef struct tc_server_stats {
   int64_t submitted, completed, rejected, batches;
   int64_t bytes_in_use, peak_bytes, waiting, in_flight;
+  int64_t canceled;
 } tc_server_stats;
 int tc_server_create(const tc_server_cfg* cfg, tc_server** out);
 /* Finishes all submitted work, then frees everything. */
 int tc_server_destroy(tc_server* s);
-/* Copies the host CSR and queues it; returns a request id >= 0, or a
- * negative TC_E* code (TC_ENOMEM if the request can never fit the budget). */
+/* Checks the host CSR (a simple undirected graph: ids in [0, nv), rows
+ * strictly increasing, no self-loops, every arc with its reverse), copies
+ * and queues it; returns a request id >= 0, or a negative TC_E* code
+ * (TC_EGRAPH for a CSR that fails the check, TC_ENOMEM if the request can
+ * never fit the budget). */
 int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h_off, const int32_t* h_ind);
 /* Waits up to timeout_ms (< 0: no limit) for request id; on completion
  * copies its nv counts to h_out (nullable), the total to h_total
  * (nullable) and the submit-to-completion time to latency_ms (nullable),
- * and forgets the request.  Returns TC_OK, 1 on timeout, or a TC_E* code. */
+ * and forgets the request.  Returns TC_OK, 1 on timeout, TC_ECANCELED for
+ * a canceled request, or a TC_E* code. */
 int tc_server_wait(tc_server* s, int64_t id, int32_t* h_out, int64_t* h_total, int32_t timeout_ms,
                    double* latency_ms);
+/* Cancels request id if it is still waiting for admission: it leaves the
+ * queue and tc_server_wait returns TC_ECANCELED for it.  Returns TC_OK, 1
+ * when the request is already admitted or done (it then completes as
+ * usual), or TC_EINVAL for an unknown id. */
+int tc_server_cancel(tc_server* s, int64_t id);
 int tc_server_get_stats(tc_server* s, tc_server_stats* st);
 
 /* Number of compute units of the context's device (for sizing). */

@@ -128,7 +128,8 @@ void           tch_batch_free(tch_batch* b);
  * most max_batch requests) whose block fits the largest free block of a
  * first-fit arena of `budget` bytes; the head is never overtaken.
  * script[i] > 0 submits a request whose share of a batch block is that many
- * bytes (tch_request_bytes minus the fixed part: see tcs::request_share);
+ * bytes, and each block adds tcs::kBatchFixed (the serving loop's blocks
+ * add tcs::kBlockFixed, which also covers the scratch's fixed part);
  * script[i] < 0 completes request -script[i]-1, and a batch's block is
  * released when all its requests have completed.  After every step batches
  * are admitted while the head fits.  admitted_at[r] gets the step at which
@@ -137,9 +138,12 @@ void           tch_batch_free(tch_batch* b);
  * (TCH_ENOMEM for a request that can never fit). */
 int tch_admission_sim(int64_t budget, int32_t max_batch, int32_t nsteps, const int64_t* script,
                       int32_t* admitted_at, int32_t* batch_of, int64_t* peak_used);
-/* Arena bytes of a request of nv vertices and nnz arcs batched alone. */
+/* Arena bytes of a request of nv vertices and nnz arcs batched alone
+ * (its input, counts and total, and the scratch of its count). */
 int64_t tch_request_bytes(int64_t nv, int64_t nnz);
-/* Exact bytes of the block of a batch of G graphs with these totals. */
+/* Exact bytes of the block of a batch of G graphs with these totals: the
+ * batched counter's packed input and output, then the count's scratch
+ * (tcs::scratch_bound bytes). */
 int64_t tch_batch_block_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
 /* Packs G graphs into a host block as the serving loop uploads it: the
  * int64 table at byte 0, offsets and ids at the regions tcs::batch_layout

@@ -13,12 +13,12 @@ def _blk(share_sum):
 
 
 def test_fifo_and_release():
-    # each 3 KiB request alone takes a 4608-byte block: two fit in 10 KiB
-    assert _blk(3 * KB) == 4608
+    # each 3 KiB request alone takes a 4864-byte block: two fit in 10 KiB
+    assert _blk(3 * KB) == 4864
     at, bo, n, peak = S.admission_sim(10 * KB, [3 * KB, 3 * KB, 3 * KB, -1, -2, -3])
     assert at.tolist() == [0, 1, 3] and n == 3
     assert bo.tolist() == [0, 1, 2]
-    assert peak == 2 * 4608
+    assert peak == 2 * 4864
 
 
 def test_waiting_requests_are_batched_together():
@@ -33,13 +33,13 @@ def test_waiting_requests_are_batched_together():
 
 def test_head_of_line_blocks_later_small_requests():
     # a large request at the head is not overtaken by a small one behind it
-    at, _, n, _ = S.admission_sim(10 * KB, [6 * KB, 6 * KB, 1 * KB, -1])
+    at, _, n, _ = S.admission_sim(11 * KB, [6 * KB, 6 * KB, 1 * KB, -1])
     assert at.tolist() == [0, 3, 3]
 
 
 def test_coalescing_makes_room_for_a_large_block():
     script = [3 * KB, 3 * KB, 3 * KB, -1, -2, 7 * KB]
-    at, _, n, _ = S.admission_sim(3 * 4608, script)
+    at, _, n, _ = S.admission_sim(3 * 4864, script)
     assert at.tolist() == [0, 1, 2, 5]
 
 
@@ -57,7 +57,7 @@ def test_never_fits_is_rejected():
 
 
 def test_completing_a_request_not_in_flight_is_an_error():
-    # request 1 still waits (two 2560-byte blocks exceed 4 KiB): not in flight
+    # request 1 still waits (two 2816-byte blocks exceed 4 KiB): not in flight
     _, _, n, _ = S.admission_sim(4 * KB, [1 * KB, 1 * KB, -2])
     assert n == -1  # TCH_EINVAL
     # completing the same request twice
@@ -164,14 +164,38 @@ def test_block_layout_within_share_bound():
         nv = rng.integers(0, 5000, G)
         nnz = rng.integers(0, 40000, G)
         exact = S.batch_block_bytes(G, int(nv.sum()), int(nnz.sum()))
-        bound = _blk(sum(S.request_share(int(a), int(b)) for a, b in zip(nv, nnz)))
+        bound = -(-(S.BLOCK_FIXED + sum(S.request_share(int(a), int(b)) for a, b in zip(nv, nnz))) // 256) * 256
         assert exact % 256 == 0 and exact <= bound
 
 
 def test_request_bytes_alignment():
     assert S.request_bytes(1000, 20000) % 256 == 0
     assert S.request_bytes(1000, 20000) >= 4 * (1001 + 20000 + 1000) + 16 + 8
-    assert S.request_bytes(1000, 20000) == _blk(S.request_share(1000, 20000))
+    assert S.request_bytes(1000, 20000) == -(-(S.BLOCK_FIXED + S.request_share(1000, 20000)) // 256) * 256
+
+
+def _orient_batch_scratch(nv, nnz):
+    """Restatement of the scratch a batched count on the oriented scheme lays
+    out (tc_capi.hip layout(): widened offsets, graph bases, counters, N+
+    bounds, the orientation, piece rows, hub rows, degrees, long rows), each
+    region rounded to 256 bytes."""
+    r = lambda b: -(-b // 256) * 256  # noqa: E731
+    return (r(8 * (nv + 1)) + r(4 * (nv + 1)) + r(64) + r(96) + r(16 * (nv + 1)) + r(4 * (nnz + 1))
+            + r(4 * (nnz // 256 + 2 + 65536)) + r(4 * (nnz // 4096 + 1)) + r(4 * (nv + 1)) + r(4 * (nnz // 512 + 2)))
+
+
+def test_scratch_bound_covers_the_count():
+    """Every batch block's scratch region holds the batched count's scratch:
+    the bound of csrc/host/admission.h against the layout restated, at the
+    sizes where its rounding and integer divisions are tightest."""
+    rng = np.random.default_rng(5)
+    sizes = [(0, 0), (1, 0), (0, 255), (1, 511), (255, 4095), (10 ** 6, 3 * 10 ** 7), (2 ** 31 - 2, 2 ** 31 - 1)]
+    sizes += [(int(a), int(b)) for a, b in zip(rng.integers(0, 10 ** 7, 500), rng.integers(0, 10 ** 9, 500))]
+    for nv, nnz in sizes:
+        blk = S.batch_block_bytes(1, nv, nnz)
+        data = S.batch_block_bytes(1, 0, 0) - (-(-(S.SCRATCH_FIXED + S.scratch_share(0, 0)) // 256) * 256)
+        assert _orient_batch_scratch(nv, nnz) <= S.SCRATCH_FIXED + S.scratch_share(nv, nnz), (nv, nnz)
+        assert blk >= data
 
 
 def _pack_block(graphs):

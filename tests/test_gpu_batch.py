@@ -96,6 +96,99 @@ def test_single_graph_capture(counter):
     cap.close()
 
 
+def test_capture_owns_its_scratch(counter):
+    """A recorded count keeps scratch of its own: after the caller's
+    context counts a much larger graph (which grows that context's scratch),
+    and with a second recorded count replayed on another stream, every
+    replay still equals the oracle."""
+    import torch
+
+    small, big = G.rmat(12, 16, seed=3), G.rmat(17, 16, seed=4)
+    sref, stot = oracle.per_vertex(small.off, small.ind)
+    off, ind = counter.to_device(small)
+    out = torch.empty(small.n, dtype=torch.int32, device="cuda")
+    tot = torch.empty(1, dtype=torch.int64, device="cuda")
+    cap = counter.capture_count(off, ind, out, tot)
+    g2 = G.erdos_renyi(5000, 40000, seed=6)
+    r2, t2 = oracle.per_vertex(g2.off, g2.ind)
+    off2, ind2 = counter.to_device(g2)
+    out2 = torch.empty(g2.n, dtype=torch.int32, device="cuda")
+    tot2 = torch.empty(1, dtype=torch.int64, device="cuda")
+    cap2 = counter.capture_count(off2, ind2, out2, tot2)
+    bout, btot = counter.count_graph(big)  # grows the caller context's scratch
+    bref, brt = oracle.per_vertex(big.off, big.ind, threads=8)
+    np.testing.assert_array_equal(bout, bref)
+    assert btot == brt
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for r in range(3):
+        out.fill_(-1)
+        out2.fill_(-1)
+        torch.cuda.synchronize()
+        cap.launch(s1)
+        cap2.launch(s2)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), sref)
+        np.testing.assert_array_equal(out2.cpu().numpy(), r2)
+        assert int(tot.item()) == stot and int(tot2.item()) == t2, f"replay {r}"
+        counter.count_graph(G.rmat(15, 16, seed=r))  # eager work between replays
+    cap.close()
+    cap2.close()
+
+
+def test_autotune_sets_a_usable_default(counter):
+    """tc_autotune_i32 with set_default: the stored default is the resolved
+    winner, so counts with cfg=None and the global-count mode still run and
+    equal the oracle."""
+    import torch
+
+    g = G.rmat(12, 16, seed=9)
+    off, ind = counter.to_device(g)
+    out = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    saved = counter.default_cfg()
+    try:
+        best, rows = counter.autotune(off, ind, out, reps=1, set_default=True)
+        assert best == counter.default_cfg()
+        ref, rt = oracle.per_vertex(g.off, g.ind)
+        o, t = counter.count_graph(g)
+        np.testing.assert_array_equal(o, ref)
+        assert t == rt
+        assert int(counter.count_total(off, ind).item()) == rt
+    finally:
+        counter.set_default_cfg(**saved)
+
+
+def test_kernel_call_shim_from_threads(counter):
+    """The kernelCall stand-in from two host threads at once (one context
+    per thread): both results equal the oracle."""
+    import threading
+
+    import torch
+
+    from gputrianglecounting_amd import tc as T
+
+    gs = [G.rmat(13, 16, seed=11), G.erdos_renyi(20000, 160000, seed=12)]
+    refs = [oracle.per_vertex(g.off, g.ind)[0] for g in gs]
+    res = [None, None]
+
+    def work(i):
+        torch.cuda.set_device(0)
+        off, ind = counter.to_device(gs[i])
+        out = torch.empty(gs[i].n, dtype=torch.int32, device="cuda")
+        for _ in range(3):
+            rc = T.kernel_call(64, 256, gs[i].n, off, ind, out, 4, 64, 2)
+            torch.cuda.synchronize()
+            assert rc == 0
+        res[i] = out.cpu().numpy()
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(2):
+        np.testing.assert_array_equal(res[i], refs[i])
+
+
 def test_autotune_all_identical(counter, tmp_path):
     import torch
 
