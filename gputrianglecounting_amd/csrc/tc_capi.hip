@@ -129,9 +129,10 @@ int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false
 
 // Scratch layout of one call.  The edge-list scheme keeps an undirected
 // edge list (at most nnz/2 edges for a symmetric CSR; in the vertex-list
-// mode every arc of a listed vertex is one, at most nnz); the oriented
-// scheme keeps the orientation in place of the input rows.
-enum class Scheme { kEdges, kList, kOrient, kOrientList };
+// mode every arc of a listed vertex is one, at most nnz, and with shadow
+// vertices at most two); the oriented scheme keeps the orientation in place
+// of the input rows.
+enum class Scheme { kEdges, kList, kListShadow, kOrient, kOrientList };
 
 Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Scheme::kEdges; }
 
@@ -201,9 +202,9 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
     return at;
   };
   const bool orient = sc == Scheme::kOrient || sc == Scheme::kOrientList, edges = !orient;
-  const bool list = sc == Scheme::kList, counts = list || sc == Scheme::kOrientList;
+  const bool list = sc == Scheme::kList || sc == Scheme::kListShadow, counts = list || sc == Scheme::kOrientList;
   L.sc = sc;
-  L.ecap = edges ? (list ? nnz + 1 : nnz / 2 + 1) : 0;
+  L.ecap = edges ? (sc == Scheme::kListShadow ? 2 * nnz + 1 : list ? nnz + 1 : nnz / 2 + 1) : 0;
   L.voff = take(8 * (size_t)(nv + 1));
   L.vbase = take(batch ? 4 * (size_t)(nv + 1) : 0);
   L.dscal = take(8 * 8);
@@ -267,7 +268,8 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
   p.qctr = reinterpret_cast<unsigned int*>(b + L.qctr);
   p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
   p.ecap = L.ecap;
-  const bool orient = L.sc == Scheme::kOrient || L.sc == Scheme::kOrientList, list = L.sc == Scheme::kList;
+  const bool orient = L.sc == Scheme::kOrient || L.sc == Scheme::kOrientList;
+  const bool list = L.sc == Scheme::kList || L.sc == Scheme::kListShadow;
   const bool counts = list || L.sc == Scheme::kOrientList;
   p.orow = orient ? reinterpret_cast<int64_t*>(b + L.orow) : nullptr;
   p.oind = orient ? reinterpret_cast<int32_t*>(b + L.oind) : nullptr;
@@ -701,13 +703,20 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   return TC_OK;
 }
 
-int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
-                          int32_t nlist, const int32_t* d_list, int32_t* d_out, int64_t* d_total, const tc_cfg* cfg_in,
-                          void* stream) {
+namespace {
+// The vertex-list count (tc_count_vertices_i32) and its shadow-vertex form
+// (tc_count_vertices_lim_i32, shadow = true): the listed vertices' arcs as
+// an edge list credited to the listed end, or, for long lists without
+// shadow vertices, the whole oriented count gathered per position.
+int count_vertices(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind, int32_t nlist,
+                   const int32_t* d_list, bool shadow, int64_t lim, int32_t* d_out, int64_t* d_total,
+                   const tc_cfg* cfg_in, void* stream) {
   if (!c) return TC_EINVAL;
   if (nv < 0 || nnz < 0 || nlist < 0 || !d_off || (nnz > 0 && !d_ind) || (nlist > 0 && (!d_list || !d_out)))
     return set_err(c, TC_EINVAL, "invalid argument");
   if (nnz > INT32_MAX) return set_err(c, TC_ERANGE, "nnz >= 2^31 on the int32 path");
+  if (shadow && 2 * nnz + 1 > INT32_MAX)
+    return set_err(c, TC_ERANGE, "nnz >= 2^30 with shadow vertices (every arc may be listed twice)");
   tc_cfg cfg;
   const int vi = resolve(c, cfg_in, &cfg);
   if (vi < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
@@ -718,7 +727,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   // cheaper as the whole oriented count gathered per position: the value of
   // a listed vertex is the same 2 T(v).  Shorter lists intersect only the
   // listed vertices' arcs.
-  if (cfg.oriented && (int64_t)nlist * 8 >= nv) {
+  if (!shadow && cfg.oriented && (int64_t)nlist * 8 >= nv) {
     const Layout L = layout(nv, nnz, false, Scheme::kOrientList);
     int rc = ensure(c, L.total);
     if (rc) return rc;
@@ -740,7 +749,7 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
     c->ev_recorded = c->timing;
     return TC_OK;
   }
-  const Layout L = layout(nv, nnz, false, Scheme::kList);
+  const Layout L = layout(nv, nnz, false, shadow ? Scheme::kListShadow : Scheme::kList);
   int rc = ensure(c, L.total);
   if (rc) return rc;
   tcd::Prep p = make_prep(c->pool, L, false);
@@ -750,7 +759,8 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   if ((e = tcd::launch_fill_i32(p.vcnt, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
   if ((e = tcd::launch_mark_list(d_list, nlist, nv, p.vmark, s)) != hipSuccess) return hip_fail(c, e, "mark");
   if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-  if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 2, c->cus, s)) != hipSuccess)
+  if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, shadow ? 3 : 2, c->cus, s, shadow ? lim : INT64_MAX)) !=
+      hipSuccess)
     return hip_fail(c, e, "edge list");
   record(c, 1, s);
   rc = run_count(c, p, cfg, vi, d_ind, p.vcnt, nullptr, 1, s, true);
@@ -762,6 +772,19 @@ int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_o
   record(c, 3, s);
   c->ev_recorded = c->timing;
   return TC_OK;
+}
+}  // namespace
+
+int tc_count_vertices_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                          int32_t nlist, const int32_t* d_list, int32_t* d_out, int64_t* d_total, const tc_cfg* cfg_in,
+                          void* stream) {
+  return count_vertices(c, nv, nnz, d_off, d_ind, nlist, d_list, false, 0, d_out, d_total, cfg_in, stream);
+}
+
+int tc_count_vertices_lim_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                              int32_t nlist, const int32_t* d_list, int32_t lim, int32_t* d_out, int64_t* d_total,
+                              const tc_cfg* cfg_in, void* stream) {
+  return count_vertices(c, nv, nnz, d_off, d_ind, nlist, d_list, true, lim, d_out, d_total, cfg_in, stream);
 }
 
 int tc_batch_count_i32(tc_ctx* c, int32_t G, const int64_t* d_tab, int64_t nv_total, int64_t nnz_total,

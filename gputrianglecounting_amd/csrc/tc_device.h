@@ -107,8 +107,11 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
                              int64_t* voff, int32_t* vbase, hipStream_t s);
 // mode 0: undirected edges u < w with both degrees >= 2 (per-vertex counts)
 // mode 1: all arcs of an oriented CSR whose endpoints both have out-arcs
+// mode 2: every arc v -> w of a listed vertex (p.vmark), w != v, both
+//         degrees >= 2 (the vertex-list count)
+// mode 3: mode 2 with the arcs whose w > lim twice (shadow vertices)
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode,
-                              int cus, hipStream_t s);
+                              int cus, hipStream_t s, int64_t lim = INT64_MAX);
 // Orientation in place (k_orient_rows) into p.orow / p.oind of a CSR with
 // nnz arcs (the size p was laid out for); ind holds graph-local ids when
 // p.vbase is set.

@@ -193,12 +193,19 @@ hipError_t scan_excl(const Tin* in, const int64_t* n_dev, int64_t n_cap, int64_t
 // with d(v) >= 2 and d(w) >= 2.  mode 1 (oriented CSR): count neighbours w
 // with d+(v) >= 1 and d+(w) >= 1.  mode 2 (vertex list): every arc v -> w of
 // a marked v with w != v, d(v) >= 2 and d(w) >= 2 (the reference's per-arc
-// rule, clusteringCount.cu:295-298).
+// rule, clusteringCount.cu:295-298).  mode 3 (vertex list with shadow
+// vertices): the arcs of mode 2, those with w > lim twice (edge_copies).
 __device__ __forceinline__ bool edge_keep(int mode, int32_t vl, int32_t wl, int64_t dv, int64_t dw) {
   if (mode == 0) return wl > vl && dv >= 2 && dw >= 2;
-  if (mode == 2) return wl != vl && dv >= 2 && dw >= 2;
+  if (mode == 2 || mode == 3) return wl != vl && dv >= 2 && dw >= 2;
   return dv >= 1 && dw >= 1;
 }
+// Copies of a kept arc v -> w in the edge list.  In the shadow-vertex
+// count (clusteringCount.cu:150-177) a match x counts 1 + [x > lim]; on a
+// simple graph the pairs (w, x) of N(v) with w ~ x are the pairs (x, w), so
+// summing 1 + [x > lim] over them equals counting every arc v -> x once
+// plus once more when x > lim.
+__device__ __forceinline__ int edge_copies(int mode, int32_t wl, int64_t lim) { return mode == 3 && wl > lim ? 2 : 1; }
 __device__ __forceinline__ int64_t edge_min_deg(int mode) { return mode == 1 ? 1 : 2; }
 
 // Vertices are handed to waves in chunks of kVChunk from a counter (ctr),
@@ -215,7 +222,7 @@ __device__ __forceinline__ int64_t grab_vertices(unsigned long long* ctr) {
 __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ voff, const int32_t* __restrict__ ind,
                                                     const int32_t* __restrict__ vbase, int64_t nv, int mode,
                                                     const int32_t* __restrict__ vmark, int32_t* __restrict__ ecnt,
-                                                    unsigned long long* ctr) {
+                                                    unsigned long long* ctr, int64_t lim) {
   const int lane = threadIdx.x & 63;
   for (;;) {
     const int64_t v0 = grab_vertices(ctr);
@@ -226,7 +233,7 @@ __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ 
       const int32_t vb = vbase ? vbase[v] : 0;
       const int32_t vl = (int32_t)(v - vb);
       int32_t c = 0;
-      if (dv >= edge_min_deg(mode) && (mode != 2 || vmark[v])) {
+      if (dv >= edge_min_deg(mode) && ((mode != 2 && mode != 3) || vmark[v])) {
         for (int64_t k0 = b; k0 < e; k0 += 256) {
           int32_t wl[4];
 #pragma unroll
@@ -242,7 +249,8 @@ __global__ __launch_bounds__(256) void k_edge_count(const int64_t* __restrict__ 
             hi[r] = voff[wg + 1];
           }
 #pragma unroll
-          for (int r = 0; r < 4; ++r) c += wl[r] >= 0 && edge_keep(mode, vl, wl[r], dv, hi[r] - lo[r]);
+          for (int r = 0; r < 4; ++r)
+            c += wl[r] >= 0 && edge_keep(mode, vl, wl[r], dv, hi[r] - lo[r]) ? edge_copies(mode, wl[r], lim) : 0;
         }
       }
       c = wave_sum32(c);
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
                                                    const int32_t* __restrict__ vmark, const int64_t* __restrict__ eoff,
                                                    int64_t ecap,
                                                    int32_t* __restrict__ eu, int32_t* __restrict__ ew,
-                                                   int32_t* __restrict__ ework, unsigned long long* ctr) {
+                                                   int32_t* __restrict__ ework, unsigned long long* ctr, int64_t lim) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (;;) {
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
     const int64_t v1 = v0 + kVChunk < nv ? v0 + kVChunk : nv;
     for (int64_t v = v0; v < v1; ++v) {
       const int64_t b = voff[v], e = voff[v + 1], dv = e - b;
-      if (dv < edge_min_deg(mode) || (mode == 2 && !vmark[v])) continue;
+      if (dv < edge_min_deg(mode) || ((mode == 2 || mode == 3) && !vmark[v])) continue;
       const int32_t vb = vbase ? vbase[v] : 0;
       const int32_t vl = (int32_t)(v - vb);
       int64_t pos = eoff[v];
@@ -287,16 +295,20 @@ __global__ __launch_bounds__(256) void k_edge_fill(const int64_t* __restrict__ v
         for (int r = 0; r < 4; ++r) {
           const int64_t dw = hi[r] - lo[r];
           const bool keep = wl[r] >= 0 && edge_keep(mode, vl, wl[r], dv, dw);
-          const uint64_t m = __ballot(keep);
-          if (keep) {
-            const int64_t idx = pos + __popcll(m & lt);
-            if (idx < ecap) {
-              eu[idx] = (int32_t)v;
-              ew[idx] = (int32_t)((int64_t)vb + wl[r]);
-              ework[idx] = (int32_t)(dv + dw);
+          for (int copy = 0; copy < 2; ++copy) {  // the second copy only in the shadow-vertex mode
+            const bool put = keep && copy < edge_copies(mode, wl[r], lim);
+            const uint64_t m = __ballot(put);
+            if (put) {
+              const int64_t idx = pos + __popcll(m & lt);
+              if (idx < ecap) {
+                eu[idx] = (int32_t)v;
+                ew[idx] = (int32_t)((int64_t)vb + wl[r]);
+                ework[idx] = (int32_t)(dv + dw);
+              }
             }
+            pos += __popcll(m);
+            if (mode != 3) break;
           }
-          pos += __popcll(m);
         }
       }
     }
@@ -947,19 +959,19 @@ hipError_t launch_voff_batch(const int64_t* tab, int32_t G, const int32_t* off_c
 }
 
 hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, const int32_t* ind, int mode, int cus,
-                              hipStream_t s) {
+                              hipStream_t s, int64_t lim) {
   const unsigned wg = grid_for(nv * 64, 256, (int64_t)cus * 16);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 2, 0, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_count, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
-                       (const int32_t*)p.vmark, p.ecnt, ctr);
+                       (const int32_t*)p.vmark, p.ecnt, ctr, lim);
   e = scan_excl<int32_t>(p.ecnt, nullptr, nv, p.eoff, p.scan_tmp, s);
   if (e != hipSuccess) return e;
   if (nv > 0)
     hipLaunchKernelGGL(k_edge_fill, dim3(wg), dim3(256), 0, s, voff, ind, (const int32_t*)p.vbase, nv, mode,
-                       (const int32_t*)p.vmark, (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework, ctr + 1);
+                       (const int32_t*)p.vmark, (const int64_t*)p.eoff, p.ecap, p.eu, p.ew, p.ework, ctr + 1, lim);
   hipLaunchKernelGGL(k_edge_finish, dim3(1), dim3(64), 0, s, (const int64_t*)p.eoff, nv, p.ecap, p.dscal);
   e = scan_excl<int32_t>(p.ework, p.dscal, p.ecap, p.wpre, p.scan_tmp, s);
   if (e != hipSuccess) return e;

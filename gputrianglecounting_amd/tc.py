@@ -112,6 +112,7 @@ def lib() -> C.CDLL:
         "tc_batch_count_i32": (C.c_int, [vp, i32, vp, i64, i64, vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_count_total_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
         "tc_count_vertices_i32": (C.c_int, [vp, i32, i64, vp, vp, i32, vp, vp, vp, vp, vp]),
+        "tc_count_vertices_lim_i32": (C.c_int, [vp, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]),
         "tc_device_cus": (C.c_int, [vp]),
         "tc_last_edge_work": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "tc_copy_i32": (C.c_int, [vp, vp, vp, i64, vp]),
@@ -139,7 +140,8 @@ EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes
            "tc_count_i32", "tc_count_noalloc_i32", "tc_kernelCall_i32", "tc_count_i64", "tc_kernelCall_i64", "tc_count_total_i64", "tc_lcc_f64",
            "tc_validate_i32",
            "tc_batch_count_i32",
-           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
+           "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32",
+           "tc_count_vertices_lim_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
            "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
            "tc_server_submit", "tc_server_wait", "tc_server_cancel", "tc_server_get_stats"]
@@ -308,10 +310,12 @@ class TriangleCounter:
         self._check(rc, "tc_count_total_i32")
         return total
 
-    def count_vertices(self, off, ind, vertices, out=None, total=None, cfg=None, stream=None):
+    def count_vertices(self, off, ind, vertices, out=None, total=None, cfg=None, stream=None, lim=None):
         """Vertex-list count (the reference's vertex-list kernelCall overload):
         ``out[i]`` = the per-vertex count of ``vertices[i]`` (-1 for an id
-        outside the graph), ``total`` int64[1] = their sum."""
+        outside the graph), ``total`` int64[1] = their sum.  With ``lim``
+        (the reference's shadow-vertex intersection, clusteringCount.cu:150-177)
+        a common neighbour x counts 1 + [x > lim]."""
         torch = self._torch
         nv = int(off.shape[0]) - 1
         nnz = int(ind.shape[0])
@@ -322,6 +326,12 @@ class TriangleCounter:
             out = torch.empty(max(nl, 0), dtype=torch.int32, device=off.device)
         if total is None:
             total = torch.empty(1, dtype=torch.int64, device=off.device)
+        if lim is not None:
+            rc = lib().tc_count_vertices_lim_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, nl,
+                                                 _p(vertices) if nl else None, int(lim), _p(out) if nl else None,
+                                                 _p(total), _cfgp(cfg), _stream(stream))
+            self._check(rc, "tc_count_vertices_lim_i32")
+            return out, total
         rc = lib().tc_count_vertices_i32(self._h, nv, nnz, _p(off), _p(ind) if nnz else None, nl,
                                          _p(vertices) if nl else None, _p(out) if nl else None, _p(total),
                                          _cfgp(cfg), _stream(stream))

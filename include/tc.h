@@ -219,6 +219,22 @@ int tc_count_vertices_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d
                           int32_t nlist, const int32_t* d_list, int32_t* d_out, int64_t* d_total, const tc_cfg* cfg,
                           void* stream);
 
+/* Vertex-list count with shadow vertices: the reference's intersection
+ * with a lim argument (intersectCount, src/clusteringCount.cu:150-177, and
+ * count_triangles, :214-246; declared at include/clusteringCount.cuh:56,73),
+ * where a common neighbour x counts 1 + [x > lim] instead of 1.  d_out[i]
+ * receives, for v = d_list[i], the sum over the arcs v -> w of
+ * tc_count_vertices_i32 of the sum over x in N(v) ∩ N(w) of 1 + [x > lim];
+ * lim = -1 doubles every value, lim >= nv - 1 gives tc_count_vertices_i32's.
+ * The reference counts a match met at a thread's split of the merge path
+ * once (fixThreadWorkEdges, :108-121), so its value depends on its launch
+ * shape there; this entry point applies the rule to every match.  The
+ * reference's overload has no caller in its sources.  nnz must stay below
+ * 2^30 (TC_ERANGE): every listed arc may become two edges. */
+int tc_count_vertices_lim_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_off, const int32_t* d_ind,
+                              int32_t nlist, const int32_t* d_list, int32_t lim, int32_t* d_out, int64_t* d_total,
+                              const tc_cfg* cfg, void* stream);
+
 /* FP64 local clustering coefficient lcc[v] = out[v] / (d(v)·(d(v)-1)),
  * 0 when d(v) < 2; one IEEE division of exactly converted integers. */
 int tc_lcc_f64(tc_ctx* ctx, int32_t nv, const int32_t* d_off, const int32_t* d_out, double* d_lcc,

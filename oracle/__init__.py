@@ -46,6 +46,8 @@ def lib():
         L.oracle_per_vertex_omp.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I64P, C.c_int32]
         L.oracle_count_arc_sample.restype = C.c_int64
         L.oracle_count_arc_sample.argtypes = [_I32P, _I32P, _I32P, _I64P, C.c_int64]
+        L.oracle_vertex_list_lim.restype = None
+        L.oracle_vertex_list_lim.argtypes = [C.c_int32, _I32P, _I32P, _I32P, C.c_int32, _I64P, _I64P]
         L.oracle_vertex_list_omp.restype = None
         L.oracle_vertex_list_omp.argtypes = [C.c_int32, _I32P, _I64P, _I32P, _I32P, _I64P, _I64P, C.c_int32]
         L.oracle_lcc.restype = None
@@ -171,6 +173,17 @@ def vertex_list_omp(vertices, off, ind, threads: int = 16):
     tot = np.zeros(1, np.int64)
     lib().oracle_vertex_list_omp(vl.shape[0], _p32(vl), _p64(pre), _p32(off), _p32(ind), _p64(out), _p64(tot),
                                  int(threads))
+    return out[: vl.shape[0]], int(tot[0])
+
+
+def vertex_list_lim(vertices, off, ind, lim):
+    """Per-list-position sums of the vertex-list GPU kernel
+    (clusteringCount.cu:266-312) with the shadow-vertex intersection
+    (clusteringCount.cu:150-177): each common neighbour x adds 1 + [x > lim]."""
+    off, ind, vl = _c(off, np.int32), _c(ind, np.int32), _c(vertices, np.int32)
+    out = np.zeros(max(vl.shape[0], 1), np.int64)
+    tot = np.zeros(1, np.int64)
+    lib().oracle_vertex_list_lim(vl.shape[0], _p32(vl), _p32(off), _p32(ind), int(lim), _p64(out), _p64(tot))
     return out[: vl.shape[0]], int(tot[0])
 
 

@@ -23,6 +23,11 @@
  *   oracle_vertex_list_omp       the vertex-list overload's per-position row
  *       sums with the listed rows' arcs split over OpenMP threads (the
  *       sampled check of graphs too large for a full CPU count).
+ *   oracle_vertex_list_lim       the vertex-list GPU kernel's per-position
+ *       sums (src/clusteringCount.cu:266-312: arcs with src != dest and both
+ *       lengths >= 2) with the shadow-vertex intersection of
+ *       src/clusteringCount.cu:150-177, each match x adding 1 + (x > lim),
+ *       applied to every match of a plain merge.
  *   oracle_count_arc_sample      the per-arc intersection of clusteringCount.cpp:57-68
  *       on a caller-chosen arc list (bounded CPU baseline).
  *   oracle_lcc                   local clustering coefficient
@@ -34,7 +39,10 @@
  * duplicates and directed rows, against committed golden vectors produced
  * from it, against closed forms (K_n, wheels, friendship graphs, cycles,
  * bipartite graphs), against diag(A^3) by dense and sparse matrix products
- * and against networkx.triangles.
+ * and against networkx.triangles.  oracle_vertex_list_lim (the reference's
+ * lim overload has no caller and no CPU form to compile): lim = -1 doubles
+ * and lim >= nv - 1 equals oracle_count_vertex_list's sums, closed forms
+ * on K_n with lim = k, and a brute-force sum over vertex triples.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -103,6 +111,36 @@ void oracle_count_vertex_list(int32_t count, const int32_t* vertexList, const in
       triNE[edge] = oracle_count_intersections(src, srcLen, ind + off[src], dest, destLen, ind + off[dest]);
       sum += triNE[edge++];
     }
+  }
+  *total = sum;
+}
+
+/* src/clusteringCount.cu:266-312 with the intersection of :150-177: out[s]
+ * = sum over the arcs src -> dest of vertexList[s] (src != dest, both rows
+ * of length >= 2) of sum over x in N(src) ∩ N(dest) of 1 + (x > lim). */
+void oracle_vertex_list_lim(int32_t count, const int32_t* vertexList, const int32_t* off, const int32_t* ind,
+                            int32_t lim, int64_t* out, int64_t* total) {
+  int64_t sum = 0;
+  for (int32_t s = 0; s < count; ++s) {
+    const int32_t src = vertexList[s];
+    const int32_t srcLen = off[src + 1] - off[src];
+    const int32_t* a = ind + off[src];
+    int64_t row = 0;
+    for (int32_t it = off[src]; it < off[src + 1]; ++it) {
+      const int32_t dest = ind[it];
+      const int32_t destLen = off[dest + 1] - off[dest];
+      if (src == dest || srcLen < 2 || destLen < 2) continue;
+      const int32_t* b = ind + off[dest];
+      int32_t ka = 0, kb = 0;
+      while (ka < srcLen && kb < destLen) {
+        const int32_t comp = a[ka] < b[kb] ? -1 : a[ka] > b[kb] ? 1 : 0;
+        if (comp == 0) row += 1 + (a[ka] > lim);
+        ka += comp <= 0;
+        kb += comp >= 0;
+      }
+    }
+    out[s] = row;
+    sum += row;
   }
   *total = sum;
 }

@@ -330,6 +330,32 @@ def test_vertex_list(counter, cfg):
         assert int(tot.item()) == etot
 
 
+@pytest.mark.parametrize("cfg", [None, dict(oriented=-1, search_ratio=1), dict(algo=3), dict(algo=2), dict(algo=1)],
+                         ids=str)
+def test_vertex_list_shadow(counter, cfg):
+    """Shadow-vertex counts (a common neighbour x adds 1 + [x > lim]) of
+    short and long lists at several lim, including -1 (every value doubled)
+    and nv - 1 (the plain count), against the oracle's restatement."""
+    import torch
+
+    g = G.rmat(13, 32, seed=3)
+    off, ind = counter.to_device(g)
+    rng = np.random.default_rng(9)
+    lists = [np.array([0, 1, 2, 0, 4095, 8191, 17], np.int32), rng.integers(0, g.n, 3000).astype(np.int32),
+             np.arange(g.n, dtype=np.int32)[::-1].copy(), np.array([-3, 5, g.n, 5], np.int32),
+             np.zeros(0, np.int32)]
+    for vl in lists:
+        for lim in (-1, 0, 100, g.n // 2, g.n - 1):
+            print(f"case shadow list len={vl.shape[0]} lim={lim} cfg={cfg}", flush=True)
+            ok = (vl >= 0) & (vl < g.n)
+            exp = np.full(vl.shape[0], -1, np.int64)
+            exp[ok], etot = oracle.vertex_list_lim(vl[ok], g.off, g.ind, lim)
+            out, tot = counter.count_vertices(off, ind, torch.from_numpy(vl).to(off.device), cfg=cfg, lim=lim)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(out.cpu().numpy().astype(np.int64), exp)
+            assert int(tot.item()) == etot
+
+
 def test_partitions_sum_to_the_whole(counter):
     import torch
 

@@ -288,6 +288,68 @@ def test_vertex_list_omp():
         assert tot == int(pv[vl].astype(np.int64).sum())
 
 
+def _lim_brute(vl, off, ind, lim):
+    """Per listed v: sum over w in N(v) (w != v, |N(v)|, |N(w)| >= 2) of
+    sum over x in N(v) ∩ N(w) of 1 + [x > lim], by Python sets (rows
+    without duplicates)."""
+    out = []
+    for v in vl:
+        Nv = ind[off[v]:off[v + 1]].tolist()
+        s = 0
+        for w in Nv:
+            Nw = ind[off[w]:off[w + 1]].tolist()
+            if w == v or len(Nv) < 2 or len(Nw) < 2:
+                continue
+            s += sum(1 + (x > lim) for x in set(Nv) & set(Nw))
+        out.append(s)
+    return np.array(out, np.int64)
+
+
+def test_vertex_list_lim_pins():
+    """The shadow-vertex restatement: lim = -1 doubles and lim >= nv - 1
+    equals the plain vertex-list sums; on K_n with lim = k every vertex gets
+    (n-1)(n-2) + (n-2)(h - [v > k]) with h = n-1-k ids above k; and a
+    brute-force set sum over vertex triples on random rows with self-loops
+    and one-way arcs."""
+    g = G.rmat(11, 16, seed=2)
+    rng = np.random.default_rng(31)
+    vl = np.concatenate([rng.integers(0, g.n, 300), [0, 0, 5]]).astype(np.int32)
+    pv, _ = oracle.per_vertex(g.off, g.ind)
+    plain = pv[vl].astype(np.int64)
+    out, tot = oracle.vertex_list_lim(vl, g.off, g.ind, -1)
+    np.testing.assert_array_equal(out, 2 * plain)
+    assert tot == int(2 * plain.sum())
+    for lim in (g.n - 1, 2**31 - 1):
+        out, tot = oracle.vertex_list_lim(vl, g.off, g.ind, lim)
+        np.testing.assert_array_equal(out, plain)
+    for n in (3, 4, 7, 12):
+        c = GR.complete(n)
+        all_v = np.arange(n, dtype=np.int32)
+        for k in range(-1, n):
+            h = n - 1 - k
+            exp = np.array([(n - 1) * (n - 2) + (n - 2) * (h - (v > k)) for v in range(n)], np.int64)
+            out, _ = oracle.vertex_list_lim(all_v, c.off, c.ind, k)
+            np.testing.assert_array_equal(out, exp)
+    for t in range(40):
+        n = int(rng.integers(2, 40))
+        rows = []
+        for v in range(n):
+            r = set(rng.integers(0, n, size=int(rng.integers(0, 9))).tolist())
+            if rng.random() < 0.3:
+                r.add(v)
+            rows.append(sorted(r))
+        off = np.zeros(n + 1, np.int32)
+        for i, r in enumerate(rows):
+            off[i + 1] = off[i] + len(r)
+        ind = np.array([x for r in rows for x in r], np.int32)
+        vl = rng.integers(0, n, size=int(rng.integers(1, 2 * n))).astype(np.int32)
+        lim = int(rng.integers(-1, n + 1))
+        out, tot = oracle.vertex_list_lim(vl, off, ind, lim)
+        exp = _lim_brute(vl, off, ind, lim)
+        np.testing.assert_array_equal(out, exp)
+        assert tot == int(exp.sum())
+
+
 def test_arc_sample_equals_full_count():
     g = G.rmat(11, 16, seed=4)
     _, tot = oracle.count_arcs(g.off, g.ind)
