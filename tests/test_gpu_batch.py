@@ -1,4 +1,6 @@
 """GPU: batches, HIP-graph replay, the per-graph baseline and the autotuner."""
+import os
+
 import numpy as np
 import pytest
 
@@ -76,6 +78,83 @@ def test_graph_replay_matches_eager(counter):
         # unrelated eager work between replays
         counter.count_graph(G.rmat(8, 8, seed=r))
     graph.close()
+
+
+def test_full_mixed_batch_captured(counter):
+    """The mixed batch at full size (4096 graphs, n log-uniform in
+    [64, 65536], ef 8..32, seed 7; ~37M vertices, ~533M arcs) counted by a
+    captured HIP graph replayed twice: every per-graph total equals the
+    oracle's (a committed fixture) and the sum
+    of its vertices' counts, every count is even and at most d(d-1); the 8
+    largest graphs and 40 seeded others equal the oracle in full; 2^16
+    uniformly drawn vertices and the 16 highest-degree ones equal the
+    oracle's vertex-list values (OpenMP restatement, per graph)."""
+    import time
+
+    import torch
+
+    t0 = time.time()
+    b = G.gen_batch(G=4096, n_lo=64, n_hi=65536, ef_lo=8, ef_hi=32, seed=7)
+    print(f"case full batch: {b.count} graphs, {b.total_vertices} vertices, {b.total_arcs} arcs "
+          f"({time.time() - t0:.1f} s)", flush=True)
+    tab, off, ind = counter.batch_to_device(b)
+    out = torch.empty(b.total_vertices, dtype=torch.int32, device="cuda")
+    totals = torch.empty(b.count, dtype=torch.int64, device="cuda")
+    graph = counter.capture_batch(tab, off, ind, b.total_vertices, b.total_arcs, out, totals)
+    try:
+        graph.launch()
+        torch.cuda.synchronize()
+        first = out.clone()
+        out.fill_(-1)
+        graph.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(out, first)
+    finally:
+        graph.close()
+    h = out.cpu().numpy().astype(np.int64)
+    ht = totals.cpu().numpy()
+    del first, off, ind, tab
+    # every graph's total, by the oracle over all arcs (tests/golden/make_batch_totals.py)
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "batch4096_seed7_totals.npz"))
+    assert int(gold["vertices"]) == b.total_vertices and int(gold["arcs"]) == b.total_arcs
+    np.testing.assert_array_equal(ht, gold["totals"])
+    _check_batch_sampled(b, h, ht, n_random=40, n_vertices=1 << 16)
+
+
+def _check_batch_sampled(b, h, ht, n_random, n_vertices):
+    """Batch counts h (per global vertex) and per-graph totals ht against
+    invariants on every vertex, the oracle on the 8 largest and n_random
+    seeded graphs in full, and its vertex-list values on the 16
+    highest-degree and n_vertices uniformly drawn vertices."""
+    vb = b.tab[0::2].astype(np.int64)
+    gid = np.repeat(np.arange(b.count, dtype=np.int64), np.diff(vb))
+    loc = np.arange(b.total_vertices, dtype=np.int64) - vb[gid]
+    rows = np.arange(b.total_vertices, dtype=np.int64) + gid
+    deg = b.off[rows + 1].astype(np.int64) - b.off[rows]
+    del rows
+    assert (h % 2 == 0).all() and (h >= 0).all() and (h <= deg * (deg - 1)).all()
+    cs = np.concatenate([[0], np.cumsum(h)])
+    np.testing.assert_array_equal(cs[vb[1:]] - cs[vb[:-1]], ht)
+    assert int(ht.sum()) % 6 == 0
+    print(f"case batch: invariants hold, sum {int(ht.sum())}", flush=True)
+    rng = np.random.default_rng(2024)
+    arcs = np.diff(b.tab[1::2].astype(np.int64))
+    whole = np.unique(np.concatenate([np.argsort(-arcs, kind="stable")[:8], rng.integers(0, b.count, n_random)]))
+    for g in whole:
+        gr = b.graph(int(g))
+        ref, rt = oracle.per_vertex(gr.off, gr.ind, threads=16)
+        np.testing.assert_array_equal(h[vb[g]:vb[g + 1]], ref.astype(np.int64), err_msg=f"graph {g}")
+        assert ht[g] == rt, f"graph {g}"
+    print(f"case batch: {whole.shape[0]} whole graphs equal the oracle", flush=True)
+    sample = np.unique(np.concatenate([np.argsort(-deg, kind="stable")[:16],
+                                       rng.integers(0, b.total_vertices, n_vertices)]))
+    sg = gid[sample]
+    for part in np.split(np.arange(sample.shape[0]), np.flatnonzero(np.diff(sg)) + 1):
+        g = int(sg[part[0]])
+        gr = b.graph(g)
+        exp, _ = oracle.vertex_list_omp(loc[sample[part]].astype(np.int32), gr.off, gr.ind, threads=16)
+        np.testing.assert_array_equal(h[sample[part]], exp, err_msg=f"graph {g}")
+    print(f"case batch: {sample.shape[0]} sampled vertices equal the oracle", flush=True)
 
 
 def test_single_graph_capture(counter):

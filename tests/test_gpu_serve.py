@@ -160,3 +160,47 @@ def test_budget_bounds_device_memory():
     finally:
         srv.close()
     assert free0 - low <= budget + (32 << 20), (free0 - low) / 2 ** 20
+
+
+def test_poisson_rmat_requests_full_size():
+    """The serving workload at full size: 8 R-MAT requests (a=.57 b=.19
+    c=.19) with n log-uniform in [2^10, 2^20] (the last one at 2^20), edge
+    factor 8..32, ids >= n dropped, submitted by a producer thread at seeded
+    Poisson arrivals into a 2 GiB budget; every result equals the oracle's
+    count of that graph."""
+    rng = np.random.default_rng(77)
+    ns = [int(np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 20)))) for _ in range(7)] + [1 << 20]
+    gs = [G.rmat(int(np.ceil(np.log2(n))), int(rng.integers(8, 33)), seed=500 + i, n_keep=n)
+          for i, n in enumerate(ns)]
+    gaps = rng.exponential(0.02, len(gs))
+    print("case serve full size: n " + " ".join(str(g.n) for g in gs) + ", arcs " +
+          " ".join(str(g.nnz) for g in gs), flush=True)
+    srv = S.Server(hbm_budget=2 << 30, slots=2, max_batch_graphs=8)
+    ids = [None] * len(gs)
+
+    def producer():
+        for j, g in enumerate(gs):
+            time.sleep(float(gaps[j]))
+            ids[j] = srv.submit(g)
+
+    try:
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        res = [srv.wait(i) for i in ids]
+    finally:
+        srv.close()
+    for g, r in zip(gs, res):
+        if g.nnz < (1 << 23):
+            ref, rt = oracle.per_vertex(g.off, g.ind, threads=16)
+            np.testing.assert_array_equal(r.out, ref, err_msg=f"n={g.n}")
+            assert r.total == rt, f"n={g.n}"
+            continue
+        d = np.diff(g.off.astype(np.int64))
+        h = r.out.astype(np.int64)
+        assert (h % 2 == 0).all() and (h >= 0).all() and (h <= d * (d - 1)).all()
+        assert r.total == int(h.sum()) and r.total % 6 == 0
+        vl = np.unique(np.concatenate([np.argsort(-d, kind="stable")[:16],
+                                       rng.integers(0, g.n, 1 << 16)])).astype(np.int32)
+        exp, _ = oracle.vertex_list_omp(vl, g.off, g.ind, threads=16)
+        np.testing.assert_array_equal(h[vl], exp, err_msg=f"n={g.n}")
