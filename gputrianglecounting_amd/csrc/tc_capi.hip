@@ -64,6 +64,9 @@ tc_cfg builtin_default() {
   c.row_groups = -1;  // automatic (orient_groups)
   c.lookup = 0;       // automatic (orient_lookup)
   c.long_rows = 0;    // automatic (orient_long)
+  c.window = tcd::kOrientWindow;
+  c.orient_wg = 256;
+  c.piece = -1;       // automatic (orient_piece_base)
   return c;
 }
 
@@ -85,6 +88,9 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
     if (in->row_groups) c.row_groups = in->row_groups;
     if (in->lookup) c.lookup = in->lookup;
     if (in->long_rows) c.long_rows = in->long_rows;
+    if (in->window) c.window = in->window;
+    if (in->orient_wg) c.orient_wg = in->orient_wg;
+    if (in->piece) c.piece = in->piece;
     c.algo = in->algo;
   }
   int vi = -1;
@@ -100,6 +106,10 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
       c.row_groups != 16 && c.row_groups != 32)
     return -1;
   if (c.lookup < 0 || c.lookup > 3 || c.long_rows < 0 || c.long_rows > 2) return -1;
+  if (!tcd::orient_window_ok(c.window)) return -1;
+  if (c.orient_wg != 128 && c.orient_wg != 256) return -1;
+  // layouts size the piece rows for pieces of kOrientPiece arcs or more
+  if (c.piece != -1 && (c.piece < tcd::kOrientPiece || 2 * (int64_t)c.piece >= 65536)) return -1;
   const int lanecap = c.wg_size * c.items;
   if (c.small_max > lanecap) c.small_max = lanecap;
   if (c.algo != 0) c.oriented = 0;  // the oriented kernel is algo 0's
@@ -282,13 +292,28 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
   return p;
 }
 
+// The oriented count's launch shape for cfg at this size: partners per
+// wave, partner side, piece size (cfg.piece, or 256 arcs for single graphs
+// and 1024 for batches), lookup, long rows, workgroup size and window.
+tcd::OrientLaunch orient_launch(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch) {
+  tcd::OrientLaunch o;
+  o.groups = orient_groups(cfg, nv, nnz, batch);
+  o.side = orient_side(cfg, batch, nnz);
+  o.piece = cfg.piece > 0 ? cfg.piece : batch ? tcd::kOrientPieceBatch : tcd::kOrientPiece;
+  o.lookup = orient_lookup(cfg, nnz, batch);
+  o.lng = orient_long(cfg, nnz, batch);
+  o.wg = cfg.orient_wg;
+  o.window = cfg.window;
+  return o;
+}
+
 template <typename OT>
 hipError_t count_oriented(const tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int64_t nv, int64_t nnz, OT* out,
                           int64_t* total, hipStream_t s, bool batch) {
-  return tcd::launch_count_orient(p, nv, nnz, out, total, orient_groups(cfg, nv, nnz, batch),
-                                  orient_side(cfg, batch, nnz), c->cus, s,
-                                  batch ? tcd::kOrientPieceBatch : tcd::kOrientPiece, orient_lookup(cfg, nnz, batch),
-                                  orient_long(cfg, nnz, batch));
+  tcd::OrientLaunch o = orient_launch(cfg, nv, nnz, batch);
+  // the other windows are compiled for int32 counts only
+  if (!out || sizeof(OT) != sizeof(int32_t)) o.window = tcd::kOrientWindow;
+  return tcd::launch_count_orient(p, nv, nnz, out, total, o, c->cus, s);
 }
 
 void record(tc_ctx* c, int i, hipStream_t s) {
@@ -943,8 +968,6 @@ int private_ctx(const tc_ctx* c, tc_ctx** out) {
   return TC_OK;
 }
 
-// Warm what a captured call must not do inside the capture: scratch
-// allocation and the occupancy query of the chosen kernel variant.
 // Everything a capture must not do inside (allocation, occupancy queries)
 // done up front for a call of this size.
 int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bool batch) {
@@ -956,7 +979,7 @@ int prepare_capture(tc_ctx* c, const tc_cfg* cfg_in, int64_t nv, int64_t nnz, bo
   (void)tcd::resident_wgs2(vi, c->cus, 1);
   const Scheme sc = scheme_of(cfg);
   if (sc == Scheme::kOrient)
-    (void)tcd::resident_orient_wgs(orient_groups(cfg, nv, nnz, batch), 1, c->cus, orient_lookup(cfg, nnz, batch));
+    (void)tcd::resident_orient_wgs(orient_launch(cfg, nv, nnz, batch), 1, c->cus);
   return ensure(c, layout(nv, nnz, batch, sc).total);
 }
 
@@ -967,16 +990,57 @@ struct Candidate {
 // The oriented scheme at every partners-per-wave setting with out- and
 // then in-neighbour partners, then the edge
 // scheme's streamed kernels (algo 0 and 3) at each compiled size.
-std::vector<tc_cfg> tune_candidates() {
+// The tuner's search, in stages, each around the fastest configuration so
+// far (best): 0 = partner side x partners per wave on the oriented scheme;
+// 1 = LDS window x workgroup size of its count kernel; 2 = piece size (the
+// partition granularity); 3 = lookup (walked array or table) x long rows;
+// 4 = the edge-list scheme's streamed kernel variants x small-edge
+// threshold x level-1 ranges per workgroup.  The reference's counterpart
+// is its sweep over 8 block sizes x 6 threads per intersection
+// (src/gpurun.cu:151-172, src/param.cu:4-5).
+constexpr int kTuneStages = 5;
+std::vector<tc_cfg> tune_stage(int stage, const tc_cfg& best) {
   std::vector<tc_cfg> out;
-  for (int o : {1, 2})
-    for (int g : {2, 4, 8, 16, 32}) {
-      tc_cfg c{};
-      c.algo = 0;
-      c.oriented = o;
-      c.row_groups = g;
+  if (stage == 0) {
+    for (int o : {1, 2})
+      for (int g : {1, 2, 4, 8, 16, 32}) {
+        tc_cfg c{};
+        c.algo = 0;
+        c.oriented = o;
+        c.row_groups = g;
+        c.lookup = 1;
+        out.push_back(c);
+      }
+    return out;
+  }
+  if (stage == 1) {
+    for (int w : {tcd::kOrientWindowSmall, tcd::kOrientWindow, tcd::kOrientWindowLarge})
+      for (int wg : {128, 256}) {
+        tc_cfg c = best;
+        c.window = w;
+        c.orient_wg = wg;
+        out.push_back(c);
+      }
+    return out;
+  }
+  if (stage == 2) {
+    for (int pc : {256, 512, 1024, 2048, 4096}) {
+      tc_cfg c = best;
+      c.piece = pc;
       out.push_back(c);
     }
+    return out;
+  }
+  if (stage == 3) {
+    for (int lk : {1, 2})
+      for (int lr : {1, 2}) {
+        tc_cfg c = best;
+        c.lookup = lk;
+        c.long_rows = lr;
+        out.push_back(c);
+      }
+    return out;
+  }
   for (int i = 0; i < tcd::num_variants(); ++i) {
     const tcd::Variant v = tcd::variant(i);
     if (v.algo != 0 && v.algo != 3) continue;  // the tuner searches the streamed kernels
@@ -1021,17 +1085,19 @@ int autotune(tc_ctx* c, Run&& run, int32_t* d_out, int64_t n, int reps, int set_
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const std::vector<tc_cfg> cands = tune_candidates();
   float best_ms = 0;
-  int best_i = -1, written = 0, rc = TC_OK;
-  bool any_diff = false;
+  int written = 0, rc = TC_OK;
+  bool any_diff = false, have_best = false, first = true;
+  tc_cfg best_cfg{};
   const bool timing = c->timing;
   c->timing = false;
-  for (size_t i = 0; i < cands.size() && rc == TC_OK; ++i) {
-    rc = run(cands[i], s);
-    if (rc) break;
+  // one candidate: its counts against the first candidate's, then timed
+  auto trial = [&](const tc_cfg& cand) -> int {
+    int r = run(cand, s);
+    if (r) return r;
     int32_t h = 0;
-    if (i == 0) {
+    if (first) {
+      first = false;
       e = hipMemcpyAsync(ref, d_out, 4 * (size_t)n, hipMemcpyDeviceToDevice, s);
     } else {
       e = tcd::launch_fill_i32(flag, 1, 0, s);
@@ -1039,29 +1105,28 @@ int autotune(tc_ctx* c, Run&& run, int32_t* d_out, int64_t n, int reps, int set_
       if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-      rc = hip_fail(c, e, "autotune check");
-      break;
-    }
+    if (e != hipSuccess) return hip_fail(c, e, "autotune check");
     (void)hipEventRecord(e0, s);
-    for (int r = 0; r < reps && rc == TC_OK; ++r) rc = run(cands[i], s);
+    for (int k = 0; k < reps && r == TC_OK; ++k) r = run(cand, s);
     (void)hipEventRecord(e1, s);
-    if (rc) break;
-    if ((e = hipEventSynchronize(e1)) != hipSuccess) {
-      rc = hip_fail(c, e, "autotune timing");
-      break;
-    }
+    if (r) return r;
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) return hip_fail(c, e, "autotune timing");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     ms /= reps;
     const bool same = h == 0;
     any_diff |= !same;
-    if (results && written < n_results) results[written++] = tc_tune_result{cands[i], ms, same ? 1 : 0};
-    if (same && (best_i < 0 || ms < best_ms)) {
-      best_i = (int)i;
+    if (results && written < n_results) results[written++] = tc_tune_result{cand, ms, same ? 1 : 0};
+    if (same && (!have_best || ms < best_ms)) {
+      have_best = true;
+      best_cfg = cand;
       best_ms = ms;
     }
-  }
+    return TC_OK;
+  };
+  for (int stage = 0; stage < kTuneStages && rc == TC_OK; ++stage)
+    for (const tc_cfg& cand : tune_stage(stage, best_cfg))
+      if ((rc = trial(cand)) != TC_OK) break;
   c->timing = timing;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1069,11 +1134,11 @@ int autotune(tc_ctx* c, Run&& run, int32_t* d_out, int64_t n, int reps, int set_
   (void)hipFree(flag);
   if (n_written) *n_written = written;
   if (rc) return rc;
-  if (best_i >= 0) {
+  if (have_best) {
     // the winner as it ran: zero fields resolved against the default in
     // force while tuning, so using it later changes nothing the timing saw
     tc_cfg r{};
-    if (resolve(c, &cands[best_i], &r) < 0) return set_err(c, TC_EINVAL, "tuned configuration does not resolve");
+    if (resolve(c, &best_cfg, &r) < 0) return set_err(c, TC_EINVAL, "tuned configuration does not resolve");
     if (best) *best = r;
     if (set_default) c->def = r;
   }

@@ -52,6 +52,14 @@ constexpr int64_t kOrientPieceMin = 16;
 constexpr int64_t kOrientPieceSlack = 65536;
 static_assert(2 * kOrientPieceBatch < 65536 && kOrientPieceMin <= kOrientPiece && kOrientPiece <= kOrientPieceBatch,
               "piece sizes");
+// Ids of N+(r) per wave LDS window of the walked-array oriented kernel
+// (k_count_orient): the default, compiled for every output, and the two
+// others compiled for int32 counts.  A window holds 6 (window + 1) bytes of
+// LDS per wave.
+constexpr int kOrientWindow = 851;
+constexpr int kOrientWindowSmall = 425;
+constexpr int kOrientWindowLarge = 1701;
+inline bool orient_window_ok(int w) { return w == kOrientWindow || w == kOrientWindowSmall || w == kOrientWindowLarge; }
 // Piece size for nnz arc positions from base (kOrientPiece or
 // kOrientPieceBatch) and the grid's waves; a result below base leaves at
 // most min(waves, kOrientPieceSlack) pieces.
@@ -124,33 +132,48 @@ int resident_wgs2(int variant_idx, int cus, int which);
 // Resident workgroups of the oriented count kernel; kind 0 = the total
 // only, 1 = int32 counts, 2 = int64 counts; lookup 0 = the sorted-window
 // merge (k_count_orient), else the table lookups (k_count_hash).
-int resident_orient_wgs(int groups, int kind, int cus, int lookup = 0);
 // nwg, nwg2, nwg3: grids of the streamed kernel's wave launch, its
 // workgroup launch and the lookup kernel (persistent: resident workgroups).
 hipError_t launch_count(const CountArgs& a, int algo, int variant_idx, int nwg, int nwg2, int nwg3, hipStream_t s);
 int resident_search_wgs(int cus);
 hipError_t launch_lcc(const int32_t* off, int32_t nv, const int32_t* out, double* lcc, hipStream_t s);
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s);
+// Launch shape of the oriented count (launch_count_orient).
+//   groups  arcs of a row streamed at a time per wave (1, 2, 4, 8, 16, 32)
+//   side    1: rows are the higher vertex of each arc (partners are its
+//           in-neighbours); 0: the lower one (partners are out-neighbours);
+//           2: 1 unless the orientation of the same call listed rows of more
+//           than kHugeRow arcs (p.dscal[6]), else 0
+//   piece   base piece size in arc positions (orient_piece picks the one
+//           used); at least kOrientPieceTune for layouts sized by it
+//   lookup  0: the row window is a sorted LDS array that every lane walks
+//           (k_count_orient); 1: an LDS cuckoo table probed twice per
+//           element, the partner lists read in wave-coalesced 16-byte
+//           vectors (k_count_hash); 2: the same kernel with every window a
+//           sorted array searched per element
+//   lng     rows whose N+ is longer than the per-wave kernel's LDS window
+//           are listed (k_long_rows) and counted by workgroups sharing one
+//           table of up to 2496 ids (k_count_long) instead of in windows
+//   wg      threads per workgroup of k_count_orient (128 or 256)
+//   window  ids of N+(r) per wave LDS window of k_count_orient
+//           (kOrientWindow; kOrientWindowSmall or kOrientWindowLarge with
+//           int32 counts)
+struct OrientLaunch {
+  int groups = 4;
+  int side = 1;
+  int64_t piece = kOrientPiece;
+  int lookup = 0;
+  bool lng = false;
+  int wg = 256;
+  int window = kOrientWindow;
+};
+int resident_orient_wgs(const OrientLaunch& o, int kind, int cus);
 // Per-vertex counts (out; nullptr: the total only) over the orientation in
-// p.orow / p.oind (k_count_orient) of nnz arcs.  groups: arcs of a row
-// streamed at a time per wave (1, 2, 4, 8, 16 or 32).  side 1: rows are the
-// higher vertex of each arc (partners are its in-neighbours); 0: the lower
-// one (partners are its out-neighbours); 2: 1 unless the orientation of the
-// same call listed rows of more than kHugeRow arcs (p.dscal[6]), else 0.
-// piece: the base piece size (orient_piece picks the one used).  lookup 0:
-// the row window is a sorted LDS array that every lane walks
-// (k_count_orient); 1: an LDS cuckoo table probed twice per element, the
-// partner lists read in wave-coalesced 16-byte vectors (k_count_hash);
-// 2: the same kernel with every window a sorted array searched per element.
-// lng: rows whose N+ is longer than the per-wave kernel's LDS window are
-// listed (k_long_rows) and counted by workgroups sharing one table of up
-// to 2496 ids (k_count_long) instead of in windows by single waves.
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece, int lookup = 0,
-                               bool lng = false);
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece = kOrientPiece, int lookup = 0,
-                               bool lng = false);
+// p.orow / p.oind of nnz arcs.
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total,
+                               const OrientLaunch& o, int cus, hipStream_t s);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total,
+                               const OrientLaunch& o, int cus, hipStream_t s);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

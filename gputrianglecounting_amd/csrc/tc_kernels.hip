@@ -17,8 +17,11 @@
 // arcs of every edge; here each undirected edge is intersected once and its
 // count added to both endpoints, and the work split is by cumulative
 // intersection length, so a hub's edges spread over many workgroups.
+#include <array>
 #include <climits>
+#include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "tc_device.h"
 
@@ -988,27 +991,34 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
   }
 }
 
-// Window of N+(r) per wave, next-chunk prefetch and waves per SIMD of the
-// oriented count kernel: the window sets the LDS per workgroup (24 (AW + 1)
-// bytes), which with the registers bounds the resident waves.
-constexpr int kOrientAW = 851;
+// Next-chunk prefetch and waves per SIMD of the oriented count kernel (its
+// workgroup size is a launch argument, OrientLaunch).
 constexpr bool kOrientPF = true;
 constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
                           unsigned long long*, unsigned long long*, int, int32_t, int64_t);
-// partners per wave (groups), per-vertex output type (or the total only),
-// one piece per wave -> kernel
-template <bool PV, typename OT, bool ONE = false>
-OrientFn<OT> orient_fn(int groups) {
+template <int AW, bool PV, typename OT, bool ONE>
+OrientFn<OT> orient_fn_w(int groups) {
   switch (groups) {
-    case 1: return k_count_orient<kOrientAW, 1, kOrientPF, kOrientWPE, PV, OT, ONE>;
-    case 2: return k_count_orient<kOrientAW, 2, kOrientPF, kOrientWPE, PV, OT, ONE>;
-    case 8: return k_count_orient<kOrientAW, 8, kOrientPF, kOrientWPE, PV, OT, ONE>;
-    case 16: return k_count_orient<kOrientAW, 16, kOrientPF, kOrientWPE, PV, OT, ONE>;
-    case 32: return k_count_orient<kOrientAW, 32, kOrientPF, kOrientWPE, PV, OT, ONE>;
-    default: return k_count_orient<kOrientAW, 4, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 1: return k_count_orient<AW, 1, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 2: return k_count_orient<AW, 2, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 8: return k_count_orient<AW, 8, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 16: return k_count_orient<AW, 16, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    case 32: return k_count_orient<AW, 32, kOrientPF, kOrientWPE, PV, OT, ONE>;
+    default: return k_count_orient<AW, 4, kOrientPF, kOrientWPE, PV, OT, ONE>;
   }
+}
+// partners per wave (groups), window, per-vertex output type (or the total
+// only), one piece per wave -> kernel.  The windows other than the default
+// are compiled for int32 counts only (the tuned paths); nullptr otherwise.
+template <bool PV, typename OT, bool ONE = false>
+OrientFn<OT> orient_fn(int groups, int window) {
+  if (window == kOrientWindow) return orient_fn_w<kOrientWindow, PV, OT, ONE>(groups);
+  if (!PV || !std::is_same<OT, int32_t>::value) return nullptr;
+  if (window == kOrientWindowSmall) return orient_fn_w<kOrientWindowSmall, PV, OT, ONE>(groups);
+  if (window == kOrientWindowLarge) return orient_fn_w<kOrientWindowLarge, PV, OT, ONE>(groups);
+  return nullptr;
 }
 // Table slots per wave and window of the hash-lookup kernel: 24 TS bytes of
 // LDS per workgroup (keys and 16-bit counters of four waves); a window of AW
@@ -1068,94 +1078,105 @@ int resident_long_wgs(int groups, int kind, int cus) {
 }
 // kind: 0 = the total only, 1 = int32 counts, 2 = int64 counts; lookup 0 =
 // k_count_orient, else k_count_hash
-const void* orient_kernel(int groups, int kind, int lookup) {
+const void* orient_kernel(int groups, int kind, int lookup, int window) {
   if (lookup) {
     if (kind == 1) return reinterpret_cast<const void*>(hash_fn<true, int32_t>(groups));
     if (kind == 2) return reinterpret_cast<const void*>(hash_fn<true, int64_t>(groups));
     return reinterpret_cast<const void*>(hash_fn<false, int32_t>(groups));
   }
-  if (kind == 1) return reinterpret_cast<const void*>(orient_fn<true, int32_t>(groups));
-  if (kind == 2) return reinterpret_cast<const void*>(orient_fn<true, int64_t>(groups));
-  return reinterpret_cast<const void*>(orient_fn<false, int32_t>(groups));
+  if (kind == 1) return reinterpret_cast<const void*>(orient_fn<true, int32_t>(groups, window));
+  if (kind == 2) return reinterpret_cast<const void*>(orient_fn<true, int64_t>(groups, window));
+  return reinterpret_cast<const void*>(orient_fn<false, int32_t>(groups, window));
 }
-int resident_orient_wgs(int groups, int kind, int cus, int lookup) {
+bool orient_shape_ok(const OrientLaunch& o, int kind) {
+  return (o.wg == 128 || o.wg == 256) && (o.lookup || orient_kernel(o.groups, kind, 0, o.window)) &&
+         o.piece >= kOrientPieceMin && 2 * o.piece < 65536;
+}
+int resident_orient_wgs(const OrientLaunch& o, int kind, int cus) {
   static std::mutex mu;
-  static int occ[2][3][33] = {};
-  const int gi = groups == 1 || groups == 2 || groups == 8 || groups == 16 || groups == 32 ? groups : 4;
+  static std::map<std::array<int, 5>, int> occ;
+  const int gi = o.groups == 1 || o.groups == 2 || o.groups == 8 || o.groups == 16 || o.groups == 32 ? o.groups : 4;
   const int ki = kind == 1 || kind == 2 ? kind : 0;
-  const int li = lookup ? 1 : 0;
+  const int li = o.lookup ? 1 : 0;
+  // the table kernel has a fixed shape
+  const std::array<int, 5> key{li, ki, gi, li ? 256 : o.wg, li ? 0 : o.window};
   std::lock_guard<std::mutex> lk(mu);
-  if (!occ[li][ki][gi]) {
+  auto it = occ.find(key);
+  if (it == occ.end()) {
+    const size_t lds = li ? 0 : orient_lds_bytes(o.wg / 64, o.window, ki != 0);
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, orient_kernel(gi, ki, li), 256, 0) != hipSuccess || n < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, orient_kernel(gi, ki, li, o.window), key[3], lds) !=
+            hipSuccess ||
+        n < 1)
       n = 1;
-    occ[li][ki][gi] = n;
+    it = occ.emplace(key, n).first;
   }
-  return occ[li][ki][gi] * cus;
+  return it->second * cus;
 }
 
 template <bool PV, typename OT>
-hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total, int groups,
-                                 int side, int kind, int cus, hipStream_t s, int64_t base, int lookup, bool lng) {
-  if (base < kOrientPieceMin || 2 * base >= 65536 || nnz < 0) return hipErrorInvalidValue;
+hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total,
+                                 const OrientLaunch& o, int kind, int cus, hipStream_t s) {
+  if (!orient_shape_ok(o, kind) || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
   hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
-  const int nwg = resident_orient_wgs(groups, kind, cus, lookup);
-  const int64_t piece = orient_piece(nnz, base, 4 * (int64_t)nwg);
+  const int nwg = resident_orient_wgs(o, kind, cus);
+  const int wg = o.lookup ? 256 : o.wg;
+  const int64_t piece = orient_piece(nnz, o.piece, (wg / 64) * (int64_t)nwg);
   hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, piece,
                      p.prow);
   // a wave for every piece: the kernel that takes one piece per wave
-  const bool one = 4 * (int64_t)nwg * piece >= nnz;
+  const bool one = (wg / 64) * (int64_t)nwg * piece >= nnz;
   // rows longer than a wave's window are listed first; the table kernel
   // counts them in its first phase, the walked-array kernel leaves them to
   // k_count_long
-  const int64_t aw = lookup ? kHashLong : kOrientAW;
-  const int64_t lmax = lng ? aw : INT64_MAX;
+  const int64_t aw = o.lookup ? kHashLong : o.window;
+  const int64_t lmax = o.lng ? aw : INT64_MAX;
   unsigned long long* lq = reinterpret_cast<unsigned long long*>(p.qctr);  // [0] rows listed, [1] rows taken
   const int64_t lcap = nnz / aw + 1;
-  if (lng) {
+  if (o.lng) {
     if (!p.lrow) return hipErrorInvalidValue;
     if ((e = launch_fill_i64(reinterpret_cast<int64_t*>(p.qctr), 2, 0, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_long_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.orow, nv, aw,
                        p.lrow, lcap, lq);
   }
-  if (lookup) {
-    const HashFn<OT> fn = lng ? (one ? hash_fn<PV, OT, true, false>(groups) : hash_fn<PV, OT, false, false>(groups))
-                              : (one ? hash_fn<PV, OT, true>(groups) : hash_fn<PV, OT>(groups));
+  if (o.lookup) {
+    const HashFn<OT> fn = o.lng ? (one ? hash_fn<PV, OT, true, false>(o.groups) : hash_fn<PV, OT, false, false>(o.groups))
+                                : (one ? hash_fn<PV, OT, true>(o.groups) : hash_fn<PV, OT>(o.groups));
     hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
                        (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out,
-                       reinterpret_cast<unsigned long long*>(total), ctr, side, (int32_t)piece, lookup == 2 ? 1 : 0,
-                       lmax, (const int32_t*)p.lrow, (const unsigned long long*)lq, lcap, lq + 1);
+                       reinterpret_cast<unsigned long long*>(total), ctr, o.side, (int32_t)piece,
+                       o.lookup == 2 ? 1 : 0, lmax, (const int32_t*)p.lrow, (const unsigned long long*)lq, lcap,
+                       lq + 1);
     return hipGetLastError();  // the long rows were counted in its first phase
   } else {
-    const OrientFn<OT> fn = one ? orient_fn<PV, OT, true>(groups) : orient_fn<PV, OT>(groups);
-    hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
-                       (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out,
-                       reinterpret_cast<unsigned long long*>(total), ctr, side, (int32_t)piece, lmax);
+    const OrientFn<OT> fn = one ? orient_fn<PV, OT, true>(o.groups, o.window) : orient_fn<PV, OT>(o.groups, o.window);
+    hipLaunchKernelGGL(fn, dim3(nwg), dim3(o.wg), orient_lds_bytes(o.wg / 64, o.window, PV), s,
+                       (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind,
+                       (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total), ctr, o.side,
+                       (int32_t)piece, lmax);
   }
-  if (lng) {
-    const LongFn<OT> lf = long_fn<PV, OT>(groups);
-    hipLaunchKernelGGL(lf, dim3(resident_long_wgs(groups, kind, cus)), dim3(256), 0, s, (const int64_t*)p.voff,
+  if (o.lng) {
+    const LongFn<OT> lf = long_fn<PV, OT>(o.groups);
+    hipLaunchKernelGGL(lf, dim3(resident_long_wgs(o.groups, kind, cus)), dim3(256), 0, s, (const int64_t*)p.voff,
                        (const int64_t*)p.orow, (const int32_t*)p.oind, (const int32_t*)p.lrow,
                        (const unsigned long long*)lq, lcap, out, reinterpret_cast<unsigned long long*>(total),
-                       (const unsigned long long*)ctr, lq + 1, side);
+                       (const unsigned long long*)ctr, lq + 1, o.side);
   }
   return hipGetLastError();
 }
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece, int lookup, bool lng) {
-  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 1, cus, s, piece, lookup, lng)
-             : launch_count_orient_t<false>(p, nv, nnz, out, total, groups, side, 0, cus, s, piece, lookup, lng);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total,
+                               const OrientLaunch& o, int cus, hipStream_t s) {
+  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, o, 1, cus, s)
+             : launch_count_orient_t<false>(p, nv, nnz, out, total, o, 0, cus, s);
 }
-hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total, int groups,
-                               int side, int cus, hipStream_t s, int64_t piece, int lookup, bool lng) {
-  if (!out)
-    return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, groups, side, 0, cus, s, piece, lookup,
-                                        lng);
-  return launch_count_orient_t<true>(p, nv, nnz, out, total, groups, side, 2, cus, s, piece, lookup, lng);
+hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total,
+                               const OrientLaunch& o, int cus, hipStream_t s) {
+  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, o, 0, cus, s);
+  return launch_count_orient_t<true>(p, nv, nnz, out, total, o, 2, cus, s);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {

@@ -117,20 +117,28 @@ __device__ __forceinline__ void out_add(int64_t* out, int64_t i, int64_t v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(out + i), (unsigned long long)v);
 }
 
+// Dynamic LDS of k_count_orient for waves of aw-id windows: per wave aw + 1
+// ids and, with per-vertex counts, (aw + 1) / 2 words of 16-bit counters,
+// each wave's two arrays side by side.
+inline size_t orient_lds_bytes(int waves, int aw, bool pv) {
+  return 4 * (size_t)waves * ((size_t)aw + 1 + (pv ? ((size_t)aw + 1) / 2 : 0));
+}
+
+// AW: ids of N+(r) per wave window.  The waves per workgroup (blockDim.x /
+// 64, 2 or 4) are a launch argument and the LDS is dynamic, sized by the
+// launcher (orient_lds_bytes).
 template <int AW, int G, bool PF, int WPE, bool PV, typename OT, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
     unsigned long long* ctr, int side, int32_t P, int64_t lmax) {
-  constexpr int NW = 4;
   constexpr int CH = 8;
-  constexpr int XW = (AW + 1) / 2;  // words of 16-bit counters per wave
-  __shared__ int32_t abuf_all[NW * (AW + 1)];
-  __shared__ int32_t xcnt_all[PV ? NW * XW : 1];
+  extern __shared__ int32_t orient_lds[];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int32_t* A = abuf_all + wv * (AW + 1);
-  int32_t* X = xcnt_all + (PV ? wv * XW : 0);
+  // this wave's window, then its (AW + 1) / 2 words of 16-bit counters
+  int32_t* A = orient_lds + wv * (PV ? AW + 1 + (AW + 1) / 2 : AW + 1);
+  int32_t* X = A + AW + 1;
   const int32_t sh = (int32_t)((reinterpret_cast<uintptr_t>(oind) >> 2) & (CH - 1));
   const int32_t* ind4 = oind - sh;
   int64_t tri = 0;  // triangles found by this wave
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   for (;;) {
     int64_t b = 0;
     if (ONE) {
-      b = (int64_t)blockIdx.x * NW + wv;
+      b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
     } else {
       if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
       b = uniform64(__shfl(b, 0, 64));

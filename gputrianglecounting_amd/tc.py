@@ -38,7 +38,8 @@ class TcCfg(C.Structure):
                 ("small_max", C.c_int32), ("ranges_per_wg", C.c_int32), ("algo", C.c_int32),
                 ("chunk", C.c_int32), ("depth", C.c_int32), ("search_ratio", C.c_int32),
                 ("probe_ratio", C.c_int32), ("oriented", C.c_int32),
-                ("row_groups", C.c_int32), ("lookup", C.c_int32), ("long_rows", C.c_int32)]
+                ("row_groups", C.c_int32), ("lookup", C.c_int32), ("long_rows", C.c_int32),
+                ("window", C.c_int32), ("orient_wg", C.c_int32), ("piece", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -93,11 +93,22 @@ typedef struct tc_ctx tc_ctx;
  *                  0 (default) = automatic: 2 for single graphs from 2^28
  *                  arcs up (with row_groups 1), else 1
  *   long_rows      oriented scheme: 2 = rows whose N+ is longer than one
- *                  wave's LDS window (851 ids for lookup 1, 624 for 2 and
- *                  3) are counted by whole workgroups sharing one table of
+ *                  wave's LDS window (`window` ids for lookup 1, 624 for 2
+ *                  and 3) are counted by whole workgroups sharing one table of
  *                  up to 2496 ids (k_count_long) instead of in windows by
  *                  single waves; 1 = windows; 0 (default) = automatic: 2
- *                  for single graphs from 2^26 arcs up, else 1 */
+ *                  for single graphs from 2^26 arcs up, else 1
+ *   window         oriented scheme, lookup 1: ids of a row's N+ per wave LDS
+ *                  window: 851 (default), 425 or 1701; the int64 path and
+ *                  the total-only count always use 851.  Rows longer than
+ *                  it are walked in windows, or go to k_count_long with
+ *                  long_rows
+ *   orient_wg      oriented scheme, lookup 1: threads per workgroup of the
+ *                  count kernel (128 or 256, default 256)
+ *   piece          oriented scheme: arc positions per work piece, the unit
+ *                  the count kernel's waves take from a counter (256..16383;
+ *                  default -1 = 256 for single graphs, 1024 for batches; a
+ *                  graph with fewer pieces than waves takes smaller ones) */
 typedef struct tc_cfg {
   int32_t wg_size;
   int32_t items;
@@ -113,6 +124,9 @@ typedef struct tc_cfg {
   int32_t row_groups;
   int32_t lookup;
   int32_t long_rows;
+  int32_t window;
+  int32_t orient_wg;
+  int32_t piece;
 } tc_cfg;
 
 enum {
@@ -285,11 +299,17 @@ int tc_graph_launch(tc_graph* g, void* stream);
 int tc_graph_destroy(tc_graph* g);
 
 /* ---------------------------------------------------------------- autotuner
- * Candidates: the oriented scheme with row_groups in {2, 4, 8, 16, 32},
- * then the edge scheme (oriented = -1) at every compiled streamed kernel
- * variant (wg_size, items, ring; algo 0 and 3) times small_max in
- * {64, 256, 1024} (at most wg_size*items) times ranges_per_wg in
- * {8, 32, 128}.  Each candidate runs once, its per-vertex output is
+ * The counterpart of the reference's sweep of 8 block sizes x 6 threads
+ * per intersection (src/gpurun.cu:151-172).  Candidates, in five stages,
+ * each around the fastest identical candidate so far: (1) the oriented
+ * scheme, oriented in {1, 2} x row_groups in {1, 2, 4, 8, 16, 32} (walked
+ * array); (2) window in {425, 851, 1701} x orient_wg in {128, 256};
+ * (3) piece in {256, 512, 1024, 2048, 4096} (the partition granularity);
+ * (4) lookup in {1, 2} x long_rows in {1, 2}; (5) the edge scheme
+ * (oriented = -1) at every compiled streamed kernel variant (wg_size,
+ * items, ring; algo 0 and 3) times small_max in {64, 256, 1024} (at most
+ * wg_size*items) times ranges_per_wg in {8, 32, 128}.  Each candidate
+ * runs once, its per-vertex output is
  * compared with the first candidate's (identical must hold: the counts are
  * exact integers), then it is timed over reps launches with HIP events.
  * The fastest identical candidate is returned in *best and, when

@@ -282,3 +282,31 @@ def test_autotune_all_identical(counter, tmp_path):
     ref, _ = oracle.per_vertex(g.off, g.ind)
     np.testing.assert_array_equal(out.cpu().numpy(), ref)
     counter.set_default_cfg(**saved)
+
+
+def test_autotune_batch_over_the_oriented_knobs(counter, tmp_path):
+    """tc_autotune_batch_i32: every stage's candidates (partner side x
+    partners per wave, LDS window x workgroup size, piece size, lookup x
+    long rows, the edge scheme) give identical counts; the winner, stored as
+    the default and in a table, counts the batch equal to the oracle."""
+    import torch
+
+    b = G.gen_batch(G=96, n_lo=64, n_hi=8192, ef_lo=8, ef_hi=32, seed=13)
+    tab, off, ind = counter.batch_to_device(b)
+    out = torch.empty(b.total_vertices, dtype=torch.int32, device="cuda")
+    saved = counter.default_cfg()
+    try:
+        best, rows = AT.tune_batch(counter, "test_batch96", tab, off, ind, b.total_vertices, b.total_arcs, out,
+                                   reps=1, path=str(tmp_path / "t.json"))
+        assert rows and all(r["identical"] for r in rows)
+        seen = {(r["window"], r["orient_wg"]) for r in rows} | {("piece", r["piece"]) for r in rows}
+        assert {(425, 128), (851, 256), (1701, 256)} <= seen and ("piece", 4096) in seen
+        assert {r["lookup"] for r in rows} >= {1, 2} and any(r["oriented"] == -1 for r in rows)
+        assert best == counter.default_cfg()
+        assert AT.best_cfg("test_batch96", path=str(tmp_path / "t.json")) == {k: best[k] for k in AT.CFG_KEYS}
+        ref, rt = _per_graph_oracle(b)
+        o, t = counter.count_batch(b)
+        np.testing.assert_array_equal(o, ref)
+        np.testing.assert_array_equal(t, rt)
+    finally:
+        counter.set_default_cfg(**saved)
