@@ -124,24 +124,10 @@ inline BatchLayout batch_layout(int64_t G, int64_t nv, int64_t nnz) {
   return L;
 }
 
-// Writes graphs 0..G-1 (nv[g] vertices, nnz[g] arcs, offsets off[g][0..nv],
-// ids ind[g][0..nnz)) into the [0, upload_end) part of a host block laid
-// out by batch_layout(G, sum nv, sum nnz); returns that layout.  The table
-// is written first; the copies, which dominate, are then split by bytes
-// over up to `threads` threads (one when the block is small) — every byte
-// lands where the serial copy would put it.
-inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
-                              const int32_t* const* ind, char* blk, int threads = 8) {
+// Table rows of a batch: tab[2g] = v_base, tab[2g+1] = arc_base, row G the
+// totals.
+inline void write_tab(int32_t G, const int32_t* nv, const int64_t* nnz, int64_t* tab) {
   int64_t v = 0, a = 0;
-  for (int32_t g = 0; g < G; ++g) {
-    v += nv[g];
-    a += nnz[g];
-  }
-  const BatchLayout L = batch_layout(G, v, a);
-  int64_t* tab = reinterpret_cast<int64_t*>(blk + L.tab);
-  int32_t* offc = reinterpret_cast<int32_t*>(blk + L.off);
-  int32_t* indc = reinterpret_cast<int32_t*>(blk + L.ind);
-  v = a = 0;
   for (int32_t g = 0; g < G; ++g) {
     tab[2 * g] = v;
     tab[2 * g + 1] = a;
@@ -150,28 +136,51 @@ inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, 
   }
   tab[2 * G] = v;
   tab[2 * G + 1] = a;
-  // copy tasks: graph g's offsets and ids, each cut into pieces of at most `cut` bytes
-  struct Task {
-    char* dst;
-    const char* src;
-    size_t n;
-  };
-  std::vector<Task> tasks;
-  const size_t total = 4 * (size_t)(v + G + a);
-  const int nt = total < (size_t(16) << 20) || threads < 2 ? 1 : threads;
-  const size_t cut = nt > 1 ? (total + (size_t)nt - 1) / (size_t)nt : total + 1;
-  auto add = [&](void* dst, const void* src, size_t n) {
-    for (size_t o = 0; o < n; o += cut)
-      tasks.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, n - o < cut ? n - o : cut});
-  };
+}
+
+// Where each graph's offsets (part 0: 4(nv+1) bytes, to block byte
+// L.off + 4(v_base + g)) and ids (part 1: 4 nnz bytes, to L.ind +
+// 4 arc_base) land in a block laid out by batch_layout: the one placement
+// both the host packer (pack_block) and the serving loop's uploads follow.
+struct UploadCopy {
+  int32_t g;
+  int part;
+  size_t dst, bytes;
+};
+inline std::vector<UploadCopy> upload_plan(int32_t G, const int32_t* nv, const int64_t* nnz, const BatchLayout& L) {
+  std::vector<UploadCopy> plan;
+  plan.reserve(2 * (size_t)G);
+  int64_t v = 0, a = 0;
   for (int32_t g = 0; g < G; ++g) {
-    add(offc + tab[2 * g] + g, off[g], 4 * ((size_t)nv[g] + 1));
-    if (nnz[g]) add(indc + tab[2 * g + 1], ind[g], 4 * (size_t)nnz[g]);
+    plan.push_back({g, 0, L.off + 4 * (size_t)(v + g), 4 * ((size_t)nv[g] + 1)});
+    if (nnz[g]) plan.push_back({g, 1, L.ind + 4 * (size_t)a, 4 * (size_t)nnz[g]});
+    v += nv[g];
+    a += nnz[g];
   }
+  return plan;
+}
+
+// n copies (dst[i], src[i], bytes[i]) split by bytes over up to `threads`
+// threads (one below 16 MiB in all); every byte lands where the serial
+// copies would put it.
+struct CopyTask {
+  char* dst;
+  const char* src;
+  size_t n;
+};
+inline void copy_tasks(const std::vector<CopyTask>& in, int threads = 8) {
+  size_t total = 0;
+  for (const CopyTask& t : in) total += t.n;
+  const int nt = total < (size_t(16) << 20) || threads < 2 ? 1 : threads;
   if (nt == 1) {
-    for (const Task& t : tasks) std::memcpy(t.dst, t.src, t.n);
-    return L;
+    for (const CopyTask& t : in) std::memcpy(t.dst, t.src, t.n);
+    return;
   }
+  // tasks cut into pieces of at most `cut` bytes
+  const size_t cut = (total + (size_t)nt - 1) / (size_t)nt;
+  std::vector<CopyTask> tasks;
+  for (const CopyTask& t : in)
+    for (size_t o = 0; o < t.n; o += cut) tasks.push_back({t.dst + o, t.src + o, t.n - o < cut ? t.n - o : cut});
   // thread k copies the tasks whose bytes fall in its share, in order
   std::vector<size_t> start(tasks.size() + 1, 0);
   for (size_t i = 0; i < tasks.size(); ++i) start[i + 1] = start[i] + tasks[i].n;
@@ -193,6 +202,25 @@ inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, 
   }
   for (size_t i = i0; i < tasks.size(); ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
   for (auto& t : pool) t.join();
+}
+
+// Writes graphs 0..G-1 (nv[g] vertices, nnz[g] arcs, offsets off[g][0..nv],
+// ids ind[g][0..nnz)) into the [0, upload_end) part of a host block laid
+// out by batch_layout(G, sum nv, sum nnz); returns that layout.  The table
+// is written first, then the copies of upload_plan (threaded when large).
+inline BatchLayout pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
+                              const int32_t* const* ind, char* blk, int threads = 8) {
+  int64_t v = 0, a = 0;
+  for (int32_t g = 0; g < G; ++g) {
+    v += nv[g];
+    a += nnz[g];
+  }
+  const BatchLayout L = batch_layout(G, v, a);
+  write_tab(G, nv, nnz, reinterpret_cast<int64_t*>(blk + L.tab));
+  std::vector<CopyTask> tasks;
+  for (const UploadCopy& c : upload_plan(G, nv, nnz, L))
+    tasks.push_back({blk + c.dst, reinterpret_cast<const char*>(c.part ? ind[c.g] : off[c.g]), c.bytes});
+  copy_tasks(tasks, threads);
   return L;
 }
 

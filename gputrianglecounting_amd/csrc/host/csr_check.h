@@ -1,6 +1,5 @@
-// The simple-graph check of one CSR row, shared by the host driver
-// (tch_check_simple, graph_host.cpp) and the serving loop's admission of
-// requests (tc_server_submit).  Header-only, no HIP.
+// The simple-graph check of one CSR row (tch_check_simple and
+// tch_check_simple64, graph_host.cpp).  Header-only, no HIP.
 #pragma once
 #include <algorithm>
 #include <cstdint>
@@ -12,31 +11,19 @@ enum RowFault { kRowOk = 0, kRowRange = 1, kRowUnsorted = 2, kRowSelfLoop = 3, k
 
 // First fault of row v of a CSR whose offsets are already known to be in
 // [0, nnz] and non-decreasing: an id outside [0, n), ids not strictly
-// increasing, a self-loop, or an arc v -> w without w -> v.
-inline int row_fault(int32_t n, const int32_t* off, const int32_t* ind, int64_t v) {
-  for (int64_t k = off[v]; k < off[v + 1]; ++k) {
-    const int32_t w = ind[k];
-    if (w < 0 || w >= n) return kRowRange;
-    if (k > off[v] && ind[k - 1] >= w) return kRowUnsorted;
-    if (w == v) return kRowSelfLoop;
+// increasing, a self-loop, or an arc v -> w without w -> v.  OT and IT are
+// the offset and id types (int32 or int64).
+template <class OT, class IT>
+inline int row_fault(int64_t n, const OT* off, const IT* ind, int64_t v) {
+  for (int64_t k = off[v]; k < (int64_t)off[v + 1]; ++k) {
+    const IT w = ind[k];
+    if (w < 0 || (int64_t)w >= n) return kRowRange;
+    if (k > (int64_t)off[v] && ind[k - 1] >= w) return kRowUnsorted;
+    if ((int64_t)w == v) return kRowSelfLoop;
   }
-  for (int64_t k = off[v]; k < off[v + 1]; ++k) {
-    const int32_t w = ind[k];
-    if (!std::binary_search(ind + off[w], ind + off[w + 1], (int32_t)v)) return kRowAsym;
-  }
-  return kRowOk;
-}
-
-// 0 when off/ind form a simple undirected graph on n vertices with nnz
-// arcs (offsets from 0 to nnz, non-decreasing; every row fault-free), else
-// nonzero.
-inline int csr_fault(int32_t n, int64_t nnz, const int32_t* off, const int32_t* ind) {
-  if (n < 0 || nnz < 0 || off[0] != 0 || off[n] != nnz) return kRowRange;
-  for (int64_t v = 0; v < n; ++v)
-    if (off[v + 1] < off[v]) return kRowRange;
-  for (int64_t v = 0; v < n; ++v) {
-    const int f = row_fault(n, off, ind, v);
-    if (f) return f;
+  for (int64_t k = off[v]; k < (int64_t)off[v + 1]; ++k) {
+    const IT w = ind[k];
+    if (!std::binary_search(ind + off[w], ind + off[w + 1], (IT)v)) return kRowAsym;
   }
   return kRowOk;
 }

@@ -389,11 +389,14 @@ int row_code(int f) {
 }
 }  // namespace
 
-int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* ind) {
+extern "C++" {
+namespace {
+// tch_check_simple for either offset/id type.
+template <class OT, class IT>
+int check_simple(int64_t n, int64_t nnz, const OT* off, const IT* ind) {
   if (n < 0 || nnz < 0 || !off || (nnz > 0 && !ind)) return fail(TCH_EINVAL, "invalid argument");
-  if (nnz > INT32_MAX) return fail(TCH_ERANGE, "nnz exceeds int32");
-  if (off[0] != 0 || off[n] != nnz) return fail(TCH_ERANGE, "off[0] != 0 or off[n] != nnz");
-  for (int32_t v = 0; v < n; ++v)
+  if ((int64_t)off[0] != 0 || (int64_t)off[n] != nnz) return fail(TCH_ERANGE, "off[0] != 0 or off[n] != nnz");
+  for (int64_t v = 0; v < n; ++v)
     if (off[v + 1] < off[v]) return fail(TCH_ERANGE, "offsets decrease at vertex " + std::to_string(v));
   std::atomic<int> code{0};
   std::atomic<int64_t> where{INT64_MAX};
@@ -418,6 +421,13 @@ int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* 
   }
   return TCH_OK;
 }
+}  // namespace
+}  // extern "C++"
+
+int tch_check_simple(int32_t n, int64_t nnz, const int32_t* off, const int32_t* ind) {
+  if (nnz > INT32_MAX) return fail(TCH_ERANGE, "nnz exceeds int32");
+  return check_simple(n, nnz, off, ind);
+}
 
 tch_graph* tch_read_info(const char* path_no_ext) {
   if (!path_no_ext) return fail_null<tch_graph>("invalid argument");
@@ -429,6 +439,7 @@ tch_graph* tch_read_info(const char* path_no_ext) {
   if (!sc.next(nv1) || !sc.next(nnz) || nv1 < 1 || nnz < 0)
     return fail_null<tch_graph>(path + ": bad header");
   if (nv1 - 1 > INT32_MAX || nnz > INT32_MAX) return fail_null<tch_graph>(path + ": sizes exceed int32");
+  if ((uint64_t)nv1 + (uint64_t)nnz > buf.size() / 2 + 1) return fail_null<tch_graph>(path + ": truncated indices");
   auto* g = new (std::nothrow) tch_graph;
   if (!g) return fail_null<tch_graph>("out of memory");
   g->n = (int32_t)(nv1 - 1);
@@ -598,6 +609,174 @@ tch_graph* tch_read_bin(const char* path) {
   if (!ok) {
     delete g;
     return fail_null<tch_graph>(std::string(path) + ": truncated");
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------- int64 CSR
+// The reference binary counts with T = int64 (src/main.cu:36 runs
+// run<int64_t>; readGraph<int64_t>, src/graphRead.cpp:2-26, reads the
+// .info text into int64 arrays).  tch_graph64 holds that layout with no
+// int32 cap on the arc count; ids stay below 2^31, the device counter's
+// limit for them.
+struct tch_graph64 {
+  int64_t n = 0;
+  std::vector<int64_t> off;
+  std::vector<int64_t> ind;
+};
+
+int64_t tch_graph64_n(const tch_graph64* g) { return g ? g->n : 0; }
+int64_t tch_graph64_nnz(const tch_graph64* g) { return g ? (int64_t)g->ind.size() : 0; }
+const int64_t* tch_graph64_off(const tch_graph64* g) { return g ? g->off.data() : nullptr; }
+const int64_t* tch_graph64_ind(const tch_graph64* g) { return g ? g->ind.data() : nullptr; }
+void tch_graph64_free(tch_graph64* g) { delete g; }
+
+tch_graph64* tch_widen(const tch_graph* g) {
+  if (!g) return fail_null<tch_graph64>("invalid argument");
+  auto* h = new (std::nothrow) tch_graph64;
+  if (!h) return fail_null<tch_graph64>("out of memory");
+  h->n = g->n;
+  h->off.assign(g->off.begin(), g->off.end());
+  h->ind.assign(g->ind.begin(), g->ind.end());
+  return h;
+}
+
+int tch_check_simple64(int64_t n, int64_t nnz, const int64_t* off, const int64_t* ind) {
+  if (n > INT32_MAX) return fail(TCH_ERANGE, "vertex count exceeds int32");
+  return check_simple(n, nnz, off, ind);
+}
+
+int tch_peek_sizes(const char* path, int64_t* nv, int64_t* nnz) {
+  if (!path || !nv || !nnz) return fail(TCH_EINVAL, "invalid argument");
+  const std::string p(path);
+  const bool bin = p.size() >= 4 && p.compare(p.size() - 4, 4, ".bin") == 0;
+  const std::string file = bin ? p : p + ".info";
+  FILE* f = std::fopen(file.c_str(), "rb");
+  if (!f) return fail(TCH_EIO, "cannot read " + file);
+  char head[160] = {};
+  const size_t got = std::fread(head, 1, sizeof head - 1, f);
+  std::fclose(f);
+  if (bin) {
+    int64_t hdr[2];
+    if (got < 24 || (std::memcmp(head, "TCCSR001", 8) != 0 && std::memcmp(head, "TCCSR064", 8) != 0))
+      return fail(TCH_EIO, file + ": bad header");
+    std::memcpy(hdr, head + 8, 16);
+    *nv = hdr[0];
+    *nnz = hdr[1];
+  } else {
+    Scanner sc{head, head + got};
+    int64_t nv1;
+    if (!sc.next(nv1) || !sc.next(*nnz) || nv1 < 1) return fail(TCH_EIO, file + ": bad header");
+    *nv = nv1 - 1;
+  }
+  if (*nv < 0 || *nnz < 0) return fail(TCH_EIO, file + ": bad header");
+  return TCH_OK;
+}
+
+tch_graph64* tch_read_info64(const char* path_no_ext) {
+  if (!path_no_ext) return fail_null<tch_graph64>("invalid argument");
+  std::string path = std::string(path_no_ext) + ".info";
+  std::string buf;
+  if (!slurp(path.c_str(), buf)) return (g_err = "cannot read " + path, nullptr);
+  Scanner sc{buf.data(), buf.data() + buf.size()};
+  int64_t nv1, nnz;
+  if (!sc.next(nv1) || !sc.next(nnz) || nv1 < 1 || nnz < 0) return fail_null<tch_graph64>(path + ": bad header");
+  if (nv1 - 1 > INT32_MAX) return fail_null<tch_graph64>(path + ": vertex count exceeds int32");
+  // every number takes at least a digit and a separator: a header asking for
+  // more than the file can hold is refused before anything is allocated
+  if ((uint64_t)nv1 + (uint64_t)nnz > buf.size() / 2 + 1) return fail_null<tch_graph64>(path + ": truncated indices");
+  auto* g = new (std::nothrow) tch_graph64;
+  if (!g) return fail_null<tch_graph64>("out of memory");
+  g->n = nv1 - 1;
+  try {
+    g->off.resize((size_t)nv1);
+    g->ind.resize((size_t)nnz);
+  } catch (...) {
+    delete g;
+    return fail_null<tch_graph64>("out of memory");
+  }
+  for (int64_t i = 0; i < nv1; ++i)
+    if (!sc.next(g->off[i])) {
+      delete g;
+      return fail_null<tch_graph64>(path + ": truncated offsets");
+    }
+  for (int64_t i = 0; i < nnz; ++i)
+    if (!sc.next(g->ind[i])) {
+      delete g;
+      return fail_null<tch_graph64>(path + ": truncated indices");
+    }
+  return g;
+}
+
+int tch_write_bin64(const tch_graph64* g, const char* path) {
+  if (!g || !path) return fail(TCH_EINVAL, "invalid argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(TCH_EIO, std::string("cannot open ") + path);
+  char magic[8] = {'T', 'C', 'C', 'S', 'R', '0', '6', '4'};
+  int64_t hdr[3] = {g->n, (int64_t)g->ind.size(), 0};
+  bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 8, 3, f) == 3 &&
+            std::fwrite(g->off.data(), 8, g->off.size(), f) == g->off.size() &&
+            std::fwrite(g->ind.data(), 8, g->ind.size(), f) == g->ind.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? TCH_OK : fail(TCH_EIO, std::string("write failed: ") + path);
+}
+
+tch_graph64* tch_read_bin64(const char* path) {
+  if (!path) return fail_null<tch_graph64>("invalid argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail_null<tch_graph64>(std::string("cannot read ") + path);
+  char magic[8];
+  int64_t hdr[3];
+  const bool ok_hdr = std::fread(magic, 1, 8, f) == 8 && std::fread(hdr, 8, 3, f) == 3;
+  const bool wide = ok_hdr && std::memcmp(magic, "TCCSR064", 8) == 0;
+  const bool narrow = ok_hdr && std::memcmp(magic, "TCCSR001", 8) == 0;
+  if (!(wide || narrow) || hdr[0] < 0 || hdr[0] > INT32_MAX || hdr[1] < 0 || (narrow && hdr[1] > INT32_MAX)) {
+    std::fclose(f);
+    return fail_null<tch_graph64>(std::string(path) + ": bad header");
+  }
+  // the header may not ask for more than the file holds
+  const long body = std::ftell(f);
+  std::fseek(f, 0, SEEK_END);
+  const long fsize = std::ftell(f);
+  std::fseek(f, body, SEEK_SET);
+  const uint64_t width = wide ? 8 : 4;
+  if (fsize < body || (uint64_t)(fsize - body) < width * ((uint64_t)hdr[0] + 1 + (uint64_t)hdr[1])) {
+    std::fclose(f);
+    return fail_null<tch_graph64>(std::string(path) + ": truncated");
+  }
+  auto* g = new (std::nothrow) tch_graph64;
+  if (!g) {
+    std::fclose(f);
+    return fail_null<tch_graph64>("out of memory");
+  }
+  g->n = hdr[0];
+  bool ok = true;
+  try {
+    g->off.resize((size_t)hdr[0] + 1);
+    g->ind.resize((size_t)hdr[1]);
+  } catch (...) {
+    ok = false;
+  }
+  if (ok && wide) {
+    ok = std::fread(g->off.data(), 8, g->off.size(), f) == g->off.size() &&
+         std::fread(g->ind.data(), 8, g->ind.size(), f) == g->ind.size();
+  } else if (ok) {
+    // the int32 cache, widened in blocks
+    std::vector<int32_t> blk(1 << 20);
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+      std::vector<int64_t>& dst = pass ? g->ind : g->off;
+      for (size_t at = 0; at < dst.size() && ok;) {
+        const size_t k = std::min(blk.size(), dst.size() - at);
+        ok = std::fread(blk.data(), 4, k, f) == k;
+        for (size_t i = 0; ok && i < k; ++i) dst[at + i] = blk[i];
+        at += k;
+      }
+    }
+  }
+  std::fclose(f);
+  if (!ok) {
+    delete g;
+    return fail_null<tch_graph64>(std::string(path) + ": truncated or out of memory");
   }
   return g;
 }

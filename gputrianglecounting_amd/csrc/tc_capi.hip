@@ -442,7 +442,7 @@ int batch_count(tc_ctx* c, void* base, size_t base_bytes, int32_t G, const int64
   }
   c->last_dscal = p.dscal;
   record(c, 2, s);
-  if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s)) != hipSuccess)
+  if (d_totals && (e = tcd::launch_graph_totals(d_tab, G, d_out_cat, d_totals, s, false, nv_total)) != hipSuccess)
     return hip_fail(c, e, "totals");
   record(c, 3, s);
   c->ev_recorded = c->timing;

@@ -180,7 +180,18 @@ hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, i
                            int cus, hipStream_t s);
 hipError_t launch_mismatch(const int32_t* a, const int32_t* b, int64_t n, int32_t* cnt, hipStream_t s);
 hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
+// Per-graph totals of a batch's counts (nv_total vertices): waves of 1024
+// vertices adding into totals[g] (zeroed first); keep_marked: totals[g] < 0
+// (a graph set aside by launch_validate_batch) is left as it is and the
+// others must already be 0.  nv_total < 0: one workgroup per graph.
 hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals,
-                               hipStream_t s);
+                               hipStream_t s, bool keep_marked = false, int64_t nv_total = -1);
+// The simple-graph contract of each graph of a packed batch (nnz_total arcs)
+// whose offsets are known to run from 0 to nnz_g without decreasing:
+// totals[g] = -1 for a graph that breaks it (0 for the others), whose ids
+// are then rewritten in place to their row's own local id, so the count
+// reads none of its rows.
+hipError_t launch_validate_batch(const int64_t* tab, int32_t G, int32_t* off_cat, int32_t* ind_cat, int64_t nnz_total,
+                                 int64_t* totals, int cus, hipStream_t s);
 
 }  // namespace tcd

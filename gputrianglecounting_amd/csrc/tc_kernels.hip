@@ -808,6 +808,127 @@ __global__ __launch_bounds__(256) void k_graph_totals(const int64_t* __restrict_
   }
 }
 
+// Graph of global vertex v of a batch: the last g with v_base[g] <= v.
+__device__ __forceinline__ int32_t graph_of_vertex(const int64_t* __restrict__ tab, int32_t G, int64_t v) {
+  int32_t lo = 0, hi = G;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (tab[2 * mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Per-graph totals of a batch, added into totals[] (zeroed before; with
+// KEEP, a graph whose total is negative, set aside by k_validate_batch, is
+// left as it is).  Each wave sums 1024 consecutive vertices, 16 per lane:
+// one atomic per wave when they share a graph (the usual case), else one
+// per lane and graph.  Even a batch of a few large graphs spreads over the
+// whole device.
+template <bool KEEP>
+__global__ __launch_bounds__(256) void k_graph_sums(const int64_t* __restrict__ tab, int32_t G,
+                                                    const int32_t* __restrict__ out, int64_t nv,
+                                                    unsigned long long* __restrict__ totals) {
+  constexpr int PER = 16;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w0 = wave * 64 * PER; w0 < nv; w0 += nwaves * 64 * PER) {
+    const int64_t w1 = w0 + 64 * PER < nv ? w0 + 64 * PER : nv;
+    const int32_t g0 = graph_of_vertex(tab, G, w0), g1 = graph_of_vertex(tab, G, w1 - 1);
+    const int64_t v0 = w0 + (int64_t)lane * PER;
+    if (g0 == g1) {
+      int64_t sum = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (v0 + q < w1) sum += out[v0 + q];
+      sum = wave_sum64(sum);
+      if (lane == 0 && sum && (!KEEP || (long long)totals[g0] >= 0)) atomicAdd(totals + g0, (unsigned long long)sum);
+    } else if (v0 < w1) {
+      int32_t g = graph_of_vertex(tab, G, v0);
+      int64_t next = g + 1 < G ? tab[2 * g + 2] : nv, sum = 0;
+      for (int q = 0; q < PER && v0 + q < w1; ++q) {
+        const int64_t v = v0 + q;
+        while (v >= next) {  // the lane's run crosses into the next graph
+          if (sum && (!KEEP || (long long)totals[g] >= 0)) atomicAdd(totals + g, (unsigned long long)sum);
+          sum = 0;
+          ++g;
+          next = g + 1 < G ? tab[2 * g + 2] : nv;
+        }
+        sum += out[v];
+      }
+      if (sum && (!KEEP || (long long)totals[g] >= 0)) atomicAdd(totals + g, (unsigned long long)sum);
+    }
+  }
+}
+
+// The simple-graph contract of every graph of a packed batch (layout of
+// include/tc_host.h: graph g's local offsets at off_cat[v_base + g], its
+// local ids at ind_cat[arc_base]), one thread per arc: a graph with an id
+// outside [0, n_g), a row not strictly increasing, a self-loop or an arc
+// without its reverse gets totals[g] = -1 (cleared to 0 before).  Every
+// graph's offsets must already run from 0 to nnz_g without decreasing (the
+// caller's check); ids are tested against n_g before they index anything.
+// The arc's graph and row come from binary searches over the table and the
+// graph's offsets, which neighbouring threads share in cache; no row is
+// walked by one wave, so a hub row costs no more than its arcs.
+__global__ __launch_bounds__(256) void k_validate_batch(const int64_t* __restrict__ tab, int32_t G,
+                                                        const int32_t* __restrict__ off_cat,
+                                                        const int32_t* __restrict__ ind_cat, int64_t nnz,
+                                                        int64_t* __restrict__ totals) {
+  for (int64_t K = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; K < nnz; K += (int64_t)gridDim.x * blockDim.x) {
+    int32_t lo = 0, hi = G;  // last g with arc_base[g] <= K: the graph whose arcs hold K
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (tab[2 * mid + 1] <= K) lo = mid;
+      else hi = mid;
+    }
+    const int64_t vb = tab[2 * lo], ab = tab[2 * lo + 1];
+    const int64_t ng = tab[2 * lo + 2] - vb, mg = tab[2 * lo + 3] - ab;
+    const int32_t* off = off_cat + vb + lo;  // graph-local offsets, ng + 1 of them
+    const int32_t* ind = ind_cat + ab;       // graph-local ids, mg of them
+    const int64_t k = K - ab;                // 0 <= k < mg
+    int64_t x = 0, xh = ng;                  // row of k: the last x with off[x] <= k
+    while (xh - x > 1) {
+      const int64_t mid = (x + xh) >> 1;
+      if (off[mid] <= k) x = mid;
+      else xh = mid;
+    }
+    const int32_t w = ind[k];
+    bool bad = w < 0 || w >= ng || w == x || (k > off[x] && ind[k - 1] >= w);
+    if (!bad) {  // x in row w
+      int64_t l = off[w], h = off[w + 1];
+      const int64_t hend = h;
+      while (l < h) {
+        const int64_t mid = (l + h) >> 1;
+        if (ind[mid] < x) l = mid + 1;
+        else h = mid;
+      }
+      bad = l >= hend || ind[l] != x;
+    }
+    if (bad) totals[lo] = -1;
+  }
+}
+
+// Sets aside the graphs k_validate_batch marked: every id of every row of
+// such a graph becomes the row's own local id, so its orientation keeps no
+// arc (N+ is empty everywhere) and the count reads none of its rows.  The
+// offsets are left as they are, so every row still ends where the next
+// begins.  One workgroup per graph at a time.
+__global__ __launch_bounds__(256) void k_drop_marked(const int64_t* __restrict__ tab, int32_t G,
+                                                     const int32_t* __restrict__ off_cat, int32_t* __restrict__ ind_cat,
+                                                     const int64_t* __restrict__ totals) {
+  for (int32_t g = blockIdx.x; g < G; g += gridDim.x) {
+    if (totals[g] >= 0) continue;  // uniform per block
+    const int64_t vb = tab[2 * g], ng = tab[2 * g + 2] - vb;
+    const int32_t* off = off_cat + vb + g;
+    int32_t* ind = ind_cat + tab[2 * g + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int64_t x = wv; x < ng; x += nw)
+      for (int64_t k = off[x] + lane; k < off[x + 1]; k += 64) ind[k] = (int32_t)x;
+  }
+}
+
 // Simple-graph contract check, one wave per vertex; flag bits:
 // 1 range/offsets, 2 unsorted or duplicate, 4 self-loop, 8 missing reverse.
 __global__ __launch_bounds__(256) void k_validate(const int32_t* __restrict__ off, const int32_t* __restrict__ ind,
@@ -1305,9 +1426,32 @@ hipError_t launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals, hipStream_t s) {
+hipError_t launch_graph_totals(const int64_t* tab, int32_t G, const int32_t* out, int64_t* totals, hipStream_t s,
+                               bool keep_marked, int64_t nv_total) {
   if (G <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_graph_totals, dim3(G < 65536 ? G : 65536), dim3(256), 0, s, tab, G, out, totals);
+  if (nv_total < 0) {  // one workgroup per graph
+    if (keep_marked) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_graph_totals, dim3(G < 65536 ? G : 65536), dim3(256), 0, s, tab, G, out, totals);
+    return hipGetLastError();
+  }
+  hipError_t e = keep_marked ? hipSuccess : launch_fill_i64(totals, G, 0, s);
+  if (e != hipSuccess || nv_total == 0) return e;
+  const unsigned grid = grid_for((nv_total + 16 * 64 - 1) / (16 * 64) * 64, 256, 8192);
+  unsigned long long* t = reinterpret_cast<unsigned long long*>(totals);
+  if (keep_marked) hipLaunchKernelGGL(k_graph_sums<true>, dim3(grid), dim3(256), 0, s, tab, G, out, nv_total, t);
+  else hipLaunchKernelGGL(k_graph_sums<false>, dim3(grid), dim3(256), 0, s, tab, G, out, nv_total, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate_batch(const int64_t* tab, int32_t G, int32_t* off_cat, int32_t* ind_cat, int64_t nnz_total,
+                                 int64_t* totals, int cus, hipStream_t s) {
+  if (G <= 0) return hipSuccess;
+  hipError_t e = launch_fill_i64(totals, G, 0, s);
+  if (e != hipSuccess || nnz_total <= 0) return e;
+  hipLaunchKernelGGL(k_validate_batch, dim3(grid_for(nnz_total, 256, (int64_t)cus * 32)), dim3(256), 0, s, tab, G,
+                     (const int32_t*)off_cat, (const int32_t*)ind_cat, nnz_total, totals);
+  hipLaunchKernelGGL(k_drop_marked, dim3(G < 4096 ? G : 4096), dim3(256), 0, s, tab, G, (const int32_t*)off_cat,
+                     ind_cat, (const int64_t*)totals);
   return hipGetLastError();
 }
 

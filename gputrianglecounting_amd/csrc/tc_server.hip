@@ -1,24 +1,32 @@
 // Serving loop (declared in include/tc.h, tc_server_*).
 //
-// One worker thread per server runs the loop
-//   retire  slots whose completion event has fired: per-vertex counts and
-//           totals are handed to their requests, the batch's arena block is
-//           released, waiters are woken
+// tc_server_submit copies each request once, into a slab of a pinned host
+// pool (as large as the device arena): [offsets | ids | counts].  Large
+// requests are copied by several threads.  One worker thread per server
+// then runs the loop
+//   retire  slots whose completion event has fired: each request's counts
+//           are taken from its slab and its total from the slot, the slab
+//           and the batch's arena block are released, waiters are woken
 //   admit   the next batch (first in first out, host/admission.h): the
 //           longest prefix of the queue whose block fits the arena of
 //           hbm_budget bytes
-//   launch  on a free slot: the batch is packed into the arena block's
-//           pinned host mirror in the batched counter's layout, uploaded with
-//           one copy on the slot's stream, counted by the batched counter
-//           with its scratch in the rest of the same block and its counts
-//           copied back with one copy, then an event is recorded
+//   launch  on a free slot: the table and every request's slab regions are
+//           uploaded straight into the block's layout by DMA copies on the
+//           slot's stream (no host packing), the batch is checked and
+//           counted by the batched counter with its scratch in the rest of
+//           the same block, the counts go back by DMA into the slabs, then
+//           an event is recorded
 // so requests that arrive while batches run are admitted as blocks free up
 // and batches on different slots overlap on the device.  Every byte of
 // device memory a batch uses is in its block, so hbm_budget bounds them all,
 // and nothing is allocated on the device after tc_server_create.
-// Requests are checked on the host when submitted (a simple undirected CSR:
-// ids in range, rows strictly increasing, no self-loops, every arc with its
-// reverse), so one malformed request can never touch another's block.
+// A request's offsets are checked on the host when it is submitted (from 0
+// to nnz, never decreasing: O(nv)); the rest of the simple-graph contract
+// (ids in range, rows strictly increasing, no self-loops, every arc with its
+// reverse) is checked on the device, per graph, before the batch is counted
+// (k_validate_batch).  A graph that breaks it has its ids rewritten to
+// self-loops of their own rows, so the count reads none of its rows and no
+// other graph of the batch is touched, and its request ends with TC_EGRAPH.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -31,8 +39,8 @@
 #include <vector>
 
 #include "host/admission.h"
-#include "host/csr_check.h"
 #include "tc.h"
+#include "tc_device.h"
 
 // tc_capi.hip: the batched count with the scratch at a given address.
 namespace tci {
@@ -45,11 +53,25 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
+// A request's slab in the pinned pool: offsets, ids and counts, each region
+// kArenaAlign aligned.
+struct Slab {
+  size_t off, ind, out, bytes;
+};
+inline Slab slab_layout(int64_t nv, int64_t nnz) {
+  Slab b;
+  b.off = 0;
+  b.ind = tcs::arena_round(4 * ((size_t)nv + 1));
+  b.out = b.ind + tcs::arena_round(4 * (size_t)nnz);
+  b.bytes = b.out + tcs::arena_round(4 * (size_t)(nv > 0 ? nv : 1));
+  return b;
+}
+
 struct Request {
   int64_t id = 0;
   int32_t nv = 0;
   int64_t nnz = 0;
-  std::vector<int32_t> off, ind;  // host copy until packed
+  size_t slab = SIZE_MAX;  // offset of its slab in the pinned pool, until retired
   bool done = false;
   int err = TC_OK;
   std::vector<int32_t> out;
@@ -61,7 +83,9 @@ struct Slot {
   tc_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  int64_t* h_tab = nullptr;  // pinned: the batch's table, then its totals
   bool busy = false;
+  bool empty = false;  // a batch without arcs: nothing launched, every count 0
   std::vector<Request*> batch;
   std::vector<int64_t> vbase;  // per request of the batch
   size_t blk_off = 0, blk_bytes = 0;
@@ -74,12 +98,13 @@ struct Slot {
 struct tc_server {
   tc_server_cfg cfg{};
   char* d_arena = nullptr;
-  char* h_mirror = nullptr;
+  char* h_pool = nullptr;  // pinned slabs of the requests not yet retired
   std::unique_ptr<tcs::Admission> adm;
+  std::unique_ptr<tcs::Arena> pool;
   std::vector<Slot> slots;
 
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_pool;
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_id = 0;
   bool stop = false;
@@ -92,71 +117,97 @@ struct tc_server {
   void release_all();
 };
 
-// Packs sl.batch into its block and queues upload, count and download on the
-// slot's stream.  Returns TC_OK or a TC_E* code (nothing then pending on the
-// stream that refers to the block except what the caller synchronises).
+// Queues the batch's upload (table and slabs, by DMA into the block's
+// layout), device check, count and download on the slot's stream.  Returns
+// TC_OK or a TC_E* code (nothing then pending on the stream that refers to
+// the block except what the caller synchronises).
 int tc_server::launch(Slot& sl) {
   const int32_t G = (int32_t)sl.batch.size();
   int64_t nv_total = 0, nnz_total = 0;
   sl.vbase.assign(G, 0);
-  for (int32_t g = 0; g < G; ++g) {
-    sl.vbase[g] = nv_total;
-    nv_total += sl.batch[g]->nv;
-    nnz_total += sl.batch[g]->nnz;
-  }
-  if (nv_total >= INT32_MAX || nnz_total > INT32_MAX) return TC_ERANGE;
   std::vector<int32_t> nv(G);
   std::vector<int64_t> nnz(G);
-  std::vector<const int32_t*> offp(G), indp(G);
   for (int32_t g = 0; g < G; ++g) {
+    sl.vbase[g] = nv_total;
     nv[g] = sl.batch[g]->nv;
     nnz[g] = sl.batch[g]->nnz;
-    offp[g] = sl.batch[g]->off.data();
-    indp[g] = sl.batch[g]->ind.data();
+    nv_total += nv[g];
+    nnz_total += nnz[g];
   }
-  if (tcs::batch_layout(G, nv_total, nnz_total).bytes > sl.blk_bytes)
-    return TC_EINVAL;  // the admission bound guarantees this never happens
-  char* h = h_mirror + sl.blk_off;
-  char* d = d_arena + sl.blk_off;
-  sl.L = tcs::pack_block(G, nv.data(), nnz.data(), offp.data(), indp.data(), h);
-  for (Request* r : sl.batch) {
-    std::vector<int32_t>().swap(r->off);
-    std::vector<int32_t>().swap(r->ind);
-  }
+  if (nv_total >= INT32_MAX || nnz_total > INT32_MAX) return TC_ERANGE;
+  sl.L = tcs::batch_layout(G, nv_total, nnz_total);
+  if (sl.L.bytes > sl.blk_bytes) return TC_EINVAL;  // the admission bound guarantees this never happens
   hipStream_t s = sl.stream;
-  if (nnz_total == 0) {  // no arcs: every count and total is zero
-    std::memset(h + sl.L.out, 0, sl.L.res_end - sl.L.out);
+  sl.empty = nnz_total == 0;
+  if (sl.empty) {  // no arcs: every count and total is zero (set at retire)
     return hipEventRecord(sl.done, s) == hipSuccess ? TC_OK : TC_EHIP;
   }
-  if (hipMemcpyAsync(d, h, sl.L.upload_end, hipMemcpyHostToDevice, s) != hipSuccess) return TC_EHIP;
-  const int rc = tci::batch_count_at(sl.ctx, d + sl.L.scr, sl.L.bytes - sl.L.scr, G,
-                                     reinterpret_cast<const int64_t*>(d + sl.L.tab), nv_total, nnz_total,
-                                     reinterpret_cast<const int32_t*>(d + sl.L.off),
-                                     reinterpret_cast<const int32_t*>(d + sl.L.ind),
-                                     reinterpret_cast<int32_t*>(d + sl.L.out),
-                                     reinterpret_cast<int64_t*>(d + sl.L.tot), s);
-  if (rc != TC_OK) return rc;
-  if (hipMemcpyAsync(h + sl.L.out, d + sl.L.out, sl.L.res_end - sl.L.out, hipMemcpyDeviceToHost, s) != hipSuccess)
+  char* d = d_arena + sl.blk_off;
+  tcs::write_tab(G, nv.data(), nnz.data(), sl.h_tab);
+  if (hipMemcpyAsync(d + sl.L.tab, sl.h_tab, 16 * ((size_t)G + 1), hipMemcpyHostToDevice, s) != hipSuccess)
     return TC_EHIP;
+  for (const tcs::UploadCopy& c : tcs::upload_plan(G, nv.data(), nnz.data(), sl.L)) {
+    const Request* r = sl.batch[c.g];
+    const Slab b = slab_layout(r->nv, r->nnz);
+    const char* src = h_pool + r->slab + (c.part ? b.ind : b.off);
+    if (hipMemcpyAsync(d + c.dst, src, c.bytes, hipMemcpyHostToDevice, s) != hipSuccess) return TC_EHIP;
+  }
+  const int64_t* d_tab = reinterpret_cast<const int64_t*>(d + sl.L.tab);
+  int32_t* d_off = reinterpret_cast<int32_t*>(d + sl.L.off);
+  int32_t* d_ind = reinterpret_cast<int32_t*>(d + sl.L.ind);
+  int32_t* d_out = reinterpret_cast<int32_t*>(d + sl.L.out);
+  int64_t* d_tot = reinterpret_cast<int64_t*>(d + sl.L.tot);
+  // graphs that break the contract: total -1, ids made harmless
+  if (tcd::launch_validate_batch(d_tab, G, d_off, d_ind, nnz_total, d_tot, tc_device_cus(sl.ctx), s) != hipSuccess)
+    return TC_EHIP;
+  const int rc = tci::batch_count_at(sl.ctx, d + sl.L.scr, sl.L.bytes - sl.L.scr, G, d_tab, nv_total, nnz_total,
+                                     d_off, d_ind, d_out, nullptr, s);
+  if (rc != TC_OK) return rc;
+  if (tcd::launch_graph_totals(d_tab, G, d_out, d_tot, s, true, nv_total) != hipSuccess) return TC_EHIP;
+  // counts into each slab, totals over the slot's table
+  for (int32_t g = 0; g < G; ++g) {
+    const Request* r = sl.batch[g];
+    if (!r->nv) continue;
+    if (hipMemcpyAsync(h_pool + r->slab + slab_layout(r->nv, r->nnz).out, d_out + sl.vbase[g], 4 * (size_t)r->nv,
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+      return TC_EHIP;
+  }
+  if (hipMemcpyAsync(sl.h_tab, d_tot, 8 * (size_t)G, hipMemcpyDeviceToHost, s) != hipSuccess) return TC_EHIP;
   if (hipEventRecord(sl.done, s) != hipSuccess) return TC_EHIP;
   return TC_OK;
 }
 
-// Hands results (or sl.err) to the batch's requests and frees its block.
+// Hands results (or sl.err) to the batch's requests and frees its block and
+// their slabs.
 void tc_server::retire(Slot& sl) {
   const auto now = clk::now();
-  const char* h = h_mirror + sl.blk_off;
-  const int32_t* out = reinterpret_cast<const int32_t*>(h + sl.L.out);
-  const int64_t* tot = reinterpret_cast<const int64_t*>(h + sl.L.tot);
+  const bool empty = sl.empty;
+  // counts out of the slabs before the lock (they are the requests' own)
+  for (size_t g = 0; g < sl.batch.size(); ++g) {
+    Request* r = sl.batch[g];
+    const int64_t tot = empty ? 0 : sl.h_tab[g];
+    if (sl.err == TC_OK && tot >= 0) {
+      if (empty) {
+        r->out.assign((size_t)r->nv, 0);
+      } else {
+        const int32_t* o = reinterpret_cast<const int32_t*>(h_pool + r->slab + slab_layout(r->nv, r->nnz).out);
+        r->out.assign(o, o + r->nv);
+      }
+      r->total = tot;
+    }
+  }
   std::lock_guard<std::mutex> lk(mu);
   for (size_t g = 0; g < sl.batch.size(); ++g) {
     Request* r = sl.batch[g];
-    if (sl.err == TC_OK) {
-      r->out.assign(out + sl.vbase[g], out + sl.vbase[g] + r->nv);
-      r->total = tot[g];
-    } else {
+    const int64_t tot = empty ? 0 : sl.h_tab[g];
+    if (sl.err != TC_OK) {
       r->err = sl.err;
+    } else if (tot < 0) {  // set aside by the device check
+      r->err = TC_EGRAPH;
+      st.rejected++;
     }
+    pool->release(r->slab, slab_layout(r->nv, r->nnz).bytes);
+    r->slab = SIZE_MAX;
     r->done = true;
     r->t_done = now;
     st.completed++;
@@ -168,6 +219,7 @@ void tc_server::retire(Slot& sl) {
   sl.err = TC_OK;
   st.bytes_in_use = (int64_t)adm->arena().used();
   cv_done.notify_all();
+  cv_pool.notify_all();
 }
 
 void tc_server::run() {
@@ -230,9 +282,10 @@ void tc_server::release_all() {
     if (sl.ctx) tc_ctx_destroy(sl.ctx);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.h_tab) (void)hipHostFree(sl.h_tab);
   }
   if (d_arena) (void)hipFree(d_arena);
-  if (h_mirror) (void)hipHostFree(h_mirror);
+  if (h_pool) (void)hipHostFree(h_pool);
 }
 
 extern "C" {
@@ -249,7 +302,7 @@ int tc_server_create(const tc_server_cfg* cfg, tc_server** out) {
   s->slots.resize(cfg->slots);
   int rc = TC_OK;
   if (hipMalloc(reinterpret_cast<void**>(&s->d_arena), budget) != hipSuccess) rc = TC_ENOMEM;
-  if (rc == TC_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_mirror), budget, hipHostMallocDefault) != hipSuccess)
+  if (rc == TC_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_pool), budget, hipHostMallocDefault) != hipSuccess)
     rc = TC_ENOMEM;
   for (auto& sl : s->slots) {
     if (rc != TC_OK) break;
@@ -257,12 +310,16 @@ int tc_server_create(const tc_server_cfg* cfg, tc_server** out) {
         hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       rc = TC_EHIP;
+    else if (hipHostMalloc(reinterpret_cast<void**>(&sl.h_tab), 16 * ((size_t)cfg->max_batch_graphs + 1),
+                           hipHostMallocDefault) != hipSuccess)
+      rc = TC_ENOMEM;
   }
   if (rc != TC_OK) {
     s->release_all();
     return rc;
   }
   s->adm = std::make_unique<tcs::Admission>(budget);
+  s->pool = std::make_unique<tcs::Arena>(budget);
   tc_server* raw = s.release();
   raw->worker = std::thread([raw] { raw->run(); });
   *out = raw;
@@ -276,6 +333,7 @@ int tc_server_destroy(tc_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_pool.notify_all();
   if (s->worker.joinable()) s->worker.join();
   s->release_all();
   delete s;
@@ -284,26 +342,47 @@ int tc_server_destroy(tc_server* s) {
 
 int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h_off, const int32_t* h_ind) {
   if (!s || nv < 0 || nnz < 0 || nnz > INT32_MAX || !h_off || (nnz > 0 && !h_ind)) return TC_EINVAL;
-  if (h_off[0] != 0 || h_off[nv] != nnz) return TC_EINVAL;
-  if (tcs::csr_fault(nv, nnz, h_off, h_ind)) {  // checked before anything is queued or launched
+  // offsets from 0 to nnz, never decreasing: every row inside the request's
+  // ids (the device checks the rest before counting)
+  bool offsets_ok = h_off[0] == 0 && h_off[nv] == nnz;
+  for (int32_t v = 0; offsets_ok && v < nv; ++v) offsets_ok = h_off[v] <= h_off[v + 1];
+  if (!offsets_ok) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->st.rejected++;
     return TC_EGRAPH;
   }
   const size_t share = tcs::request_share(nv, nnz);
+  const Slab b = slab_layout(nv, nnz);
   auto r = std::make_unique<Request>();
   r->nv = nv;
   r->nnz = nnz;
-  r->off.assign(h_off, h_off + (size_t)nv + 1);
-  r->ind.assign(h_ind, h_ind + nnz);
+  {
+    // a slab in the pinned pool; while it is full (requests queued or in
+    // flight hold it), wait for the worker to retire some
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->adm->fits_ever(share) || b.bytes > s->pool->capacity()) {
+      s->st.rejected++;
+      return TC_ENOMEM;
+    }
+    for (;;) {
+      if (s->stop) return TC_EINVAL;
+      r->slab = s->pool->alloc(b.bytes);
+      if (r->slab != SIZE_MAX) break;
+      s->cv_pool.wait(lk);
+    }
+  }
+  // the one host copy of the request (several threads when large)
+  char* slab = s->h_pool + r->slab;
+  tcs::copy_tasks({{slab + b.off, reinterpret_cast<const char*>(h_off), 4 * ((size_t)nv + 1)},
+                   {slab + b.ind, reinterpret_cast<const char*>(h_ind), 4 * (size_t)nnz}});
   r->t_submit = clk::now();
   int64_t id;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) return TC_EINVAL;
-    if (!s->adm->fits_ever(share)) {
-      s->st.rejected++;
-      return TC_ENOMEM;
+    if (s->stop) {
+      s->pool->release(r->slab, b.bytes);
+      s->cv_pool.notify_all();
+      return TC_EINVAL;
     }
     id = s->next_id++;
     r->id = id;
@@ -346,8 +425,9 @@ int tc_server_cancel(tc_server* s, int64_t id) {
   r->done = true;
   r->err = TC_ECANCELED;
   r->t_done = clk::now();
-  std::vector<int32_t>().swap(r->off);
-  std::vector<int32_t>().swap(r->ind);
+  s->pool->release(r->slab, slab_layout(r->nv, r->nnz).bytes);
+  r->slab = SIZE_MAX;
+  s->cv_pool.notify_all();
   s->st.canceled++;
   s->st.waiting = (int64_t)s->adm->waiting();
   s->cv_done.notify_all();

@@ -72,6 +72,17 @@ def host_lib() -> C.CDLL:
         "tch_read_partition": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_I32P)]),
         "tch_partition_members": (C.c_int, [C.c_int32, _I32P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_I32P)]),
         "tch_free": (None, [vp]),
+        "tch_graph64_n": (C.c_int64, [vp]),
+        "tch_graph64_nnz": (C.c_int64, [vp]),
+        "tch_graph64_off": (_I64P, [vp]),
+        "tch_graph64_ind": (_I64P, [vp]),
+        "tch_graph64_free": (None, [vp]),
+        "tch_read_info64": (vp, [C.c_char_p]),
+        "tch_read_bin64": (vp, [C.c_char_p]),
+        "tch_write_bin64": (C.c_int, [vp, C.c_char_p]),
+        "tch_widen": (vp, [vp]),
+        "tch_check_simple64": (C.c_int, [C.c_int64, C.c_int64, _I64P, _I64P]),
+        "tch_peek_sizes": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -233,6 +244,64 @@ def partition_members(part, p: int) -> np.ndarray:
     if rc != 0:
         raise HostError(_err())
     return _take_i32(ptr, cnt.value)
+
+
+def _from_handle64(h):
+    L = host_lib()
+    if not h:
+        raise HostError(_err())
+    try:
+        n = L.tch_graph64_n(h)
+        nnz = L.tch_graph64_nnz(h)
+        off = np.ctypeslib.as_array(L.tch_graph64_off(h), shape=(n + 1,)).copy()
+        ind = np.ctypeslib.as_array(L.tch_graph64_ind(h), shape=(nnz,)).copy() if nnz else np.zeros(0, np.int64)
+    finally:
+        L.tch_graph64_free(h)
+    return off, ind
+
+
+def read_info64(path_no_ext: str):
+    """``.info`` read as the reference binary reads it (readGraph<int64_t>,
+    graphRead.cpp:2-26, run by main.cu:36): int64 ``(off, ind)``, any arc count."""
+    return _from_handle64(host_lib().tch_read_info64(path_no_ext.encode()))
+
+
+def read_bin64(path: str):
+    """int64 ``(off, ind)`` from the int64 binary cache, or the int32 one widened."""
+    return _from_handle64(host_lib().tch_read_bin64(path.encode()))
+
+
+def write_bin64(g: Graph, path: str) -> None:
+    """The int64 binary cache ("TCCSR064") of ``g``."""
+    h = _to_handle(g)
+    L = host_lib()
+    w = L.tch_widen(h)
+    L.tch_graph_free(h)
+    if not w:
+        raise HostError(_err())
+    try:
+        rc = L.tch_write_bin64(w, path.encode())
+        if rc != 0:
+            raise HostError(_err(), rc)
+    finally:
+        L.tch_graph64_free(w)
+
+
+def check_simple64(off, ind) -> int:
+    """tch_check_simple on an int64 CSR: 0 if simple and symmetric, else a TCH_E* code."""
+    o = np.ascontiguousarray(off, dtype=np.int64)
+    i = np.ascontiguousarray(ind, dtype=np.int64)
+    return int(host_lib().tch_check_simple64(o.shape[0] - 1, i.shape[0], o.ctypes.data_as(_I64P),
+                                             i.ctypes.data_as(_I64P)))
+
+
+def peek_sizes(path: str):
+    """(n, nnz) from the header of ``path + '.info'``, or of ``path`` if it ends in ``.bin``."""
+    n, nnz = C.c_int64(), C.c_int64()
+    rc = host_lib().tch_peek_sizes(path.encode(), C.byref(n), C.byref(nnz))
+    if rc != 0:
+        raise HostError(_err(), rc)
+    return int(n.value), int(nnz.value)
 
 
 def read_bin(path: str) -> Graph:

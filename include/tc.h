@@ -135,7 +135,7 @@ enum {
   TC_ERANGE = -2,    /* sizes beyond the int32 path (nnz >= 2^31) */
   TC_EHIP = -3,      /* a HIP runtime call failed (tc_last_error has the text) */
   TC_ENOMEM = -4,    /* device scratch could not be allocated */
-  TC_EGRAPH = -5,    /* tc_validate_i32, tc_server_submit: input is not a simple undirected CSR */
+  TC_EGRAPH = -5,    /* tc_validate_i32, tc_server_submit/_wait: input is not a simple undirected CSR */
   TC_ECANCELED = -6  /* tc_server_wait: the request was canceled (tc_server_cancel) */
 };
 
@@ -357,10 +357,12 @@ int tc_copy_i32(tc_ctx* ctx, int32_t* d_dst, const int32_t* d_src, int64_t n, vo
  * offsets, ids), its counts and totals, and the scratch of its count, and
  * stays allocated until the batch completes: hbm_budget bounds all the
  * device memory of the requests in flight (the server allocates nothing on
- * the device after tc_server_create).  A batch is packed into a pinned host
- * mirror of its block, uploaded with one copy, counted and copied back on
- * one of `slots` launch slots (each with its own stream), so batches overlap
- * on the device while later requests wait for memory.  Counts are those of
+ * the device after tc_server_create).  Each request is copied once, at
+ * submit, into a pinned host slab; a batch's slabs are uploaded by DMA
+ * straight into its block's layout, checked, counted and copied back on one
+ * of `slots` launch slots (each with its own stream), so batches overlap on
+ * the device while later requests wait for memory.  tc_server_destroy must
+ * not race with other calls on the same server.  Counts are those of
  * tc_count_i32 on each graph alone.  No reference counterpart (the
  * reference counts one graph per process, gpurun.cu:2-56). */
 typedef struct tc_server tc_server;
@@ -379,11 +381,17 @@ typedef struct tc_server_stats {
 int tc_server_create(const tc_server_cfg* cfg, tc_server** out);
 /* Finishes all submitted work, then frees everything. */
 int tc_server_destroy(tc_server* s);
-/* Checks the host CSR (a simple undirected graph: ids in [0, nv), rows
- * strictly increasing, no self-loops, every arc with its reverse), copies
- * and queues it; returns a request id >= 0, or a negative TC_E* code
- * (TC_EGRAPH for a CSR that fails the check, TC_ENOMEM if the request can
- * never fit the budget). */
+/* Checks the host CSR's offsets (h_off[0] = 0, never decreasing, h_off[nv]
+ * = nnz), copies it into a slab of the server's pinned host pool (hbm_budget
+ * bytes; while the requests queued or in flight fill it, the call waits for
+ * the worker to retire some) and queues it; returns a request id >= 0, or a negative
+ * TC_E* code (TC_EGRAPH for offsets that fail the check, TC_ENOMEM if the
+ * request can never fit the budget).  The rest of the simple-graph contract
+ * (ids in [0, nv), rows strictly increasing, no self-loops, every arc with
+ * its reverse) is checked on the device before the request's batch is
+ * counted: a request that breaks it is set aside (the count reads none of
+ * its rows and no other request's results change) and tc_server_wait
+ * returns TC_EGRAPH for it. */
 int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h_off, const int32_t* h_ind);
 /* Waits up to timeout_ms (< 0: no limit) for request id; on completion
  * copies its nv counts to h_out (nullable), the total to h_total

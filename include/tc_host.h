@@ -93,6 +93,32 @@ void tch_free(void* p);
 int        tch_write_bin(const tch_graph* g, const char* path);
 tch_graph* tch_read_bin(const char* path);
 
+/* ---------------------------------------------------------------- int64 CSR
+ * The reference binary counts with T = int64 (src/main.cu:36 runs
+ * run<int64_t>; readGraph<int64_t>, src/graphRead.cpp:2-26, reads .info
+ * into int64 arrays).  tch_graph64 holds off[n+1] and ind[nnz] as int64 with
+ * no cap on nnz; n and every id stay below 2^31 (tc_count_i64's limit).
+ *   tch_read_info64   the .info text (appends ".info"), any nnz
+ *   tch_read_bin64    the int64 cache ("TCCSR064", n, nnz, reserved, then
+ *                     int64 off and ind) or the int32 cache, widened
+ *   tch_write_bin64   writes the int64 cache
+ *   tch_widen         an int32 graph as int64
+ *   tch_check_simple64  tch_check_simple for the int64 layout
+ *   tch_peek_sizes    n and nnz from the header of <path>.info, or of path
+ *                     itself when it ends in ".bin", without reading the rest */
+typedef struct tch_graph64 tch_graph64;
+int64_t        tch_graph64_n(const tch_graph64* g);
+int64_t        tch_graph64_nnz(const tch_graph64* g);
+const int64_t* tch_graph64_off(const tch_graph64* g);
+const int64_t* tch_graph64_ind(const tch_graph64* g);
+void           tch_graph64_free(tch_graph64* g);
+tch_graph64*   tch_read_info64(const char* path_no_ext);
+tch_graph64*   tch_read_bin64(const char* path);
+int            tch_write_bin64(const tch_graph64* g, const char* path);
+tch_graph64*   tch_widen(const tch_graph* g);
+int            tch_check_simple64(int64_t n, int64_t nnz, const int64_t* off, const int64_t* ind);
+int            tch_peek_sizes(const char* path, int64_t* nv, int64_t* nnz);
+
 /* ---------------------------------------------------------------- batches
  * G graphs packed into one concatenated CSR:
  *   tab[2*(G+1)]  int64: tab[2g] = v_base (sum of earlier n), tab[2g+1] =

@@ -642,9 +642,11 @@ def test_tc_main_partition_run(tmp_path):
 
 
 def test_tc_main_sweep(tmp_path):
-    """tc_main <graph> 1 <blocks> (no block size): the sweep file
-    <graph>.o.<blocks> has one row per oriented partners-per-wave setting and
-    per edge-scheme variant, every total the oracle's."""
+    """tc_main <graph> 1 <blocks> (no block size): the reference's sweep file
+    <graph>.o.<blocks> beside the graph (gpurun.cu:146-172), one row
+    "bs tpi allocMs execMs sumMs total" per legacy shape in the reference's
+    order (param.cu:4-5), every total the oracle's; --variants writes one
+    row per oriented partners-per-wave setting and edge-scheme variant."""
     import os
     import subprocess
 
@@ -652,10 +654,51 @@ def test_tc_main_sweep(tmp_path):
     base = str(tmp_path / "g")
     g.write_info(base + ".info")
     exe = os.path.join(os.path.dirname(T.__file__), "_lib", "tc_main")
-    print("case tc_main sweep", flush=True)
+    _, rtot = _ref(g)
+    print("case tc_main sweep (reference rows)", flush=True)
     r = subprocess.run([exe, base, "1", "8"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     rows = [line.split("\t") for line in open(base + ".o.8").read().splitlines()]  # next to the graph
-    _, rtot = _ref(g)
+    shapes = [(b, t) for b in (32, 64, 96, 128, 192, 256, 512, 1024) for t in (1, 2, 4, 8, 16, 32)]
+    assert [(int(row[0]), int(row[1])) for row in rows] == shapes
+    assert all(len(row) == 6 and int(row[5]) == rtot for row in rows)
+    assert all(abs(float(row[4]) - float(row[2]) - float(row[3])) < 1e-3 * max(1.0, float(row[4])) for row in rows)
+    print("case tc_main sweep --variants", flush=True)
+    r = subprocess.run([exe, "--variants", base, "1", "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split("\t") for line in open(base + ".o.8").read().splitlines()]
     assert {row[0] for row in rows} == {"oriented", "edges"} and len(rows) == 14
     assert all(int(row[-1]) == rtot for row in rows)
+
+
+def test_tc_main_int64(tmp_path):
+    """tc_main --int64: the graph read as the reference binary reads it
+    (readGraph<int64_t>, run<int64_t>: main.cu:36) and counted by
+    tc_count_i64; .info and the int64 binary cache, single run and sweep,
+    totals the oracle's."""
+    import os
+    import subprocess
+
+    g = G.rmat(12, 16, seed=10)
+    base = str(tmp_path / "g")
+    g.write_info(base + ".info")
+    G.write_bin64(g, base + ".bin")
+    exe = os.path.join(os.path.dirname(T.__file__), "_lib", "tc_main")
+    _, rtot = _ref(g)
+    for spec in (base, base + ".bin"):
+        print(f"case tc_main --int64 {os.path.basename(spec)}", flush=True)
+        r = subprocess.run([exe, "--int64", spec, "1", "8", "256", "4"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        nv, total = r.stdout.split()[:2]
+        assert int(nv) == g.n and int(total) == rtot
+    print("case tc_main --int64 sweep", flush=True)
+    r = subprocess.run([exe, "--int64", base, "1", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split("\t") for line in open(base + ".o.4").read().splitlines()]
+    assert len(rows) == 48 and all(int(row[5]) == rtot for row in rows)
+    # int64 inputs are not normalised: a one-way arc is refused before any launch
+    bad = tmp_path / "bad.info"
+    bad.write_text("4\t3\n0\t1\t2\t3\n1\t2\t0\n")
+    r = subprocess.run([exe, "--int64", str(tmp_path / "bad"), "1", "8", "256", "4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1 and "not a simple undirected CSR" in r.stderr

@@ -254,3 +254,73 @@ def test_partition_file(tmp_path):
         f.write("4\n0 1 2\n")
     with pytest.raises(G.HostError):
         G.read_partition(base, 5)  # truncated
+
+
+# ---------------------------------------------------------------- int64 CSR
+def test_info64_matches_int32_reader_and_reference(tmp_path):
+    """tch_read_info64 (the reference binary's readGraph<int64_t>,
+    graphRead.cpp:2-26 via main.cu:36) reads the same numbers as the int32
+    reader and as the compiled reference reader."""
+    g = G.rmat(10, 16, seed=3)
+    base = str(tmp_path / "g")
+    g.write_info(base + ".info")
+    off, ind = G.read_info64(base)
+    assert off.dtype == np.int64 and ind.dtype == np.int64
+    np.testing.assert_array_equal(off, g.off.astype(np.int64))
+    np.testing.assert_array_equal(ind, g.ind.astype(np.int64))
+    assert G.peek_sizes(base) == (g.n, g.nnz)
+    assert G.check_simple64(off, ind) == 0
+    L = oracle.ref_lib()
+    if L is None:
+        pytest.skip("oracle/_ref not built")
+    import ctypes as C
+
+    nv, nnz = C.c_int32(0), C.c_int32(0)
+    roff = np.zeros(g.n + 1, np.int32)
+    rind = np.zeros(g.nnz, np.int32)
+    assert L.tcref_read_info(base.encode(), C.byref(nv), C.byref(nnz), oracle._p32(roff), roff.shape[0],
+                             oracle._p32(rind), rind.shape[0]) == 0
+    np.testing.assert_array_equal(off, roff.astype(np.int64))
+    np.testing.assert_array_equal(ind, rind.astype(np.int64))
+
+
+def test_info64_offsets_beyond_int32(tmp_path):
+    """A header and offsets past 2^31 are read as written (the int32 reader
+    refuses them); peek_sizes reports them without reading the body."""
+    big = 2**31 + 5
+    p = tmp_path / "h.info"
+    p.write_text(f"3\t{big}\n0\t{big - 3}\t{big}\n")
+    assert G.peek_sizes(str(tmp_path / "h")) == (2, big)
+    with pytest.raises(G.HostError, match="int32"):
+        G.read_info(str(tmp_path / "h"))
+    # the body is short of 2^31 ids: refused from the file size, before any
+    # allocation
+    with pytest.raises(G.HostError, match="truncated indices"):
+        G.read_info64(str(tmp_path / "h"))
+
+
+def test_bin64_round_trip_and_widening(tmp_path):
+    g = G.geometric(5000, 6.0, seed=4)
+    p64 = str(tmp_path / "g64.bin")
+    G.write_bin64(g, p64)
+    off, ind = G.read_bin64(p64)
+    np.testing.assert_array_equal(off, g.off.astype(np.int64))
+    np.testing.assert_array_equal(ind, g.ind.astype(np.int64))
+    assert G.peek_sizes(p64) == (g.n, g.nnz)
+    p32 = str(tmp_path / "g32.bin")
+    g.write_bin(p32)
+    off2, ind2 = G.read_bin64(p32)  # the int32 cache, widened
+    np.testing.assert_array_equal(off2, off)
+    np.testing.assert_array_equal(ind2, ind)
+    with open(p64, "r+b") as f:  # truncate the body
+        f.truncate(32 + 8 * (g.n + 1) + 8)
+    with pytest.raises(G.HostError, match="truncated"):
+        G.read_bin64(p64)
+
+
+def test_check_simple64_codes():
+    assert G.check_simple64([0, 1, 2], [1, 0]) == 0
+    assert G.check_simple64([0, 2, 3], [0, 1, 0]) == -5  # self-loop
+    assert G.check_simple64([0, 1, 1], [1]) == -6  # arc without reverse
+    assert G.check_simple64([0, 1, 2], [7, 0]) == -3  # id out of range
+    assert G.check_simple64([0, 2, 3, 4], [2, 1, 0, 0]) == -4  # row 0 = [2, 1]
