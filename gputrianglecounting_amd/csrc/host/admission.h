@@ -10,12 +10,16 @@
 // scratch of the batch's count: the arena bounds all the device memory of
 // the batches in flight.
 #pragma once
+#include <atomic>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <deque>
 #include <iterator>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <system_error>
 #include <thread>
 #include <vector>
@@ -203,6 +207,101 @@ inline void copy_tasks(const std::vector<CopyTask>& in, int threads = 8) {
   for (size_t i = i0; i < tasks.size(); ++i) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].n);
   for (auto& t : pool) t.join();
 }
+
+// A fixed set of threads for large copies (the serving loop's submits), so
+// a copy is split without starting threads for every call.  run() cuts the
+// copies into pieces of 1 MiB that the pool's threads and the caller take
+// in turn, and returns when all are copied.  Each call is a job object of
+// its own, held by every thread working on it, so a thread still looking at
+// a finished job only finds its piece counter exhausted.
+class CopyPool {
+ public:
+  explicit CopyPool(int threads) {
+    for (int k = 0; k < threads; ++k) {
+      try {
+        th_.emplace_back([this] { loop(); });
+      } catch (const std::system_error&) {
+        break;  // fewer helpers; the callers copy what they do not
+      }
+    }
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void run(const std::vector<CopyTask>& in) {
+    size_t total = 0;
+    for (const CopyTask& t : in) total += t.n;
+    if (th_.empty() || total < (size_t(2) << 20)) {
+      for (const CopyTask& t : in) std::memcpy(t.dst, t.src, t.n);
+      return;
+    }
+    constexpr size_t kCut = size_t(1) << 20;
+    auto job = std::make_shared<Job>();
+    for (const CopyTask& t : in)
+      for (size_t o = 0; o < t.n; o += kCut)
+        job->tasks.push_back({t.dst + o, t.src + o, t.n - o < kCut ? t.n - o : kCut});
+    job->left = job->tasks.size();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      cur_ = job;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work(*job);
+    {
+      std::unique_lock<std::mutex> lk(job->m);
+      job->cv.wait(lk, [&] { return job->left == 0; });
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    if (cur_ == job) cur_.reset();
+  }
+
+ private:
+  struct Job {
+    std::vector<CopyTask> tasks;
+    std::atomic<size_t> next{0};
+    size_t left = 0;  // pieces not yet copied (under m)
+    std::mutex m;
+    std::condition_variable cv;
+  };
+  // Copies pieces of the job until none is left; the last one wakes the caller.
+  static void work(Job& j) {
+    size_t mine = 0;
+    for (size_t i; (i = j.next.fetch_add(1)) < j.tasks.size();) {
+      std::memcpy(j.tasks[i].dst, j.tasks[i].src, j.tasks[i].n);
+      ++mine;
+    }
+    if (!mine) return;
+    std::lock_guard<std::mutex> lk(j.m);
+    j.left -= mine;
+    if (j.left == 0) j.cv.notify_all();
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::shared_ptr<Job> j;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (gen_ != seen && cur_); });
+        if (stop_) return;
+        seen = gen_;
+        j = cur_;
+      }
+      work(*j);
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::shared_ptr<Job> cur_;  // the newest job (under mu_)
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
 
 // Writes graphs 0..G-1 (nv[g] vertices, nnz[g] arcs, offsets off[g][0..nv],
 // ids ind[g][0..nnz)) into the [0, upload_end) part of a host block laid

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 struct tch_graph {
@@ -915,6 +916,23 @@ int64_t tch_pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const i
   }
   if ((int64_t)tcs::batch_layout(G, v, a).upload_end > block_bytes) return fail(TCH_EINVAL, "block too small");
   return (int64_t)tcs::pack_block(G, nv, nnz, off, ind, static_cast<char*>(block)).upload_end;
+}
+
+int tch_copy_pool_run(int32_t threads, int32_t callers, int32_t rounds, int32_t n, void* const* dst,
+                      const void* const* src, const int64_t* bytes) {
+  if (threads < 0 || callers < 1 || rounds < 0 || n < 0 || (n > 0 && (!dst || !src || !bytes)))
+    return fail(TCH_EINVAL, "invalid argument");
+  tcs::CopyPool pool(threads);
+  std::vector<std::thread> cs;
+  for (int32_t c = 0; c < callers; ++c)
+    cs.emplace_back([&, c] {
+      std::vector<tcs::CopyTask> t;
+      for (int32_t i = 0; i < n; ++i)
+        t.push_back({static_cast<char*>(dst[(size_t)c * n + i]), static_cast<const char*>(src[i]), (size_t)bytes[i]});
+      for (int32_t r = 0; r < rounds; ++r) pool.run(t);
+    });
+  for (auto& t : cs) t.join();
+  return TCH_OK;
 }
 
 int tch_admission_sim(int64_t budget, int32_t max_batch, int32_t nsteps, const int64_t* script,

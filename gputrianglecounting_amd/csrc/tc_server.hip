@@ -101,6 +101,7 @@ struct tc_server {
   char* h_pool = nullptr;  // pinned slabs of the requests not yet retired
   std::unique_ptr<tcs::Admission> adm;
   std::unique_ptr<tcs::Arena> pool;
+  std::unique_ptr<tcs::CopyPool> copier;  // threads for large submits
   std::vector<Slot> slots;
 
   std::mutex mu;
@@ -320,6 +321,7 @@ int tc_server_create(const tc_server_cfg* cfg, tc_server** out) {
   }
   s->adm = std::make_unique<tcs::Admission>(budget);
   s->pool = std::make_unique<tcs::Arena>(budget);
+  s->copier = std::make_unique<tcs::CopyPool>(7);
   tc_server* raw = s.release();
   raw->worker = std::thread([raw] { raw->run(); });
   *out = raw;
@@ -371,10 +373,10 @@ int64_t tc_server_submit(tc_server* s, int32_t nv, int64_t nnz, const int32_t* h
       s->cv_pool.wait(lk);
     }
   }
-  // the one host copy of the request (several threads when large)
+  // the one host copy of the request (the copy pool's threads when large)
   char* slab = s->h_pool + r->slab;
-  tcs::copy_tasks({{slab + b.off, reinterpret_cast<const char*>(h_off), 4 * ((size_t)nv + 1)},
-                   {slab + b.ind, reinterpret_cast<const char*>(h_ind), 4 * (size_t)nnz}});
+  s->copier->run({{slab + b.off, reinterpret_cast<const char*>(h_off), 4 * ((size_t)nv + 1)},
+                  {slab + b.ind, reinterpret_cast<const char*>(h_ind), 4 * (size_t)nnz}});
   r->t_submit = clk::now();
   int64_t id;
   {

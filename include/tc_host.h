@@ -177,6 +177,12 @@ int64_t tch_batch_block_bytes(int32_t G, int64_t nv_total, int64_t nnz_total);
  * size) or a negative TCH_E* code. */
 int64_t tch_pack_block(int32_t G, const int32_t* nv, const int64_t* nnz, const int32_t* const* off,
                        const int32_t* const* ind, void* block, int64_t block_bytes);
+/* The serving loop's copy pool (tcs::CopyPool of `threads` threads) run by
+ * `callers` threads at once, each copying `rounds` times n buffers
+ * (src[i] -> dst[c][i], bytes[i] bytes; dst[c] is caller c's own set).
+ * Returns 0, or a negative TCH_E* code. */
+int tch_copy_pool_run(int32_t threads, int32_t callers, int32_t rounds, int32_t n, void* const* dst,
+                      const void* const* src, const int64_t* bytes);
 
 enum {
   TCH_OK = 0,

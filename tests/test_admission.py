@@ -264,3 +264,30 @@ def test_server_block_parallel_copy():
     np.testing.assert_array_equal(blk[:tab_bytes].view(np.int64), b.tab)
     np.testing.assert_array_equal(blk[off_at:off_at + off_bytes].view(np.int32), b.off)
     np.testing.assert_array_equal(blk[ind_at:n].view(np.int32), b.ind)
+
+
+def test_copy_pool_concurrent_callers():
+    """The serving loop's copy pool (large submits are split over its
+    threads): several callers at once, many rounds, buffers from a few bytes
+    to several MiB (pieces of 1 MiB, the 2 MiB inline threshold on both
+    sides); every destination byte equals its source."""
+    import ctypes as C
+
+    from gputrianglecounting_amd import graph as G
+
+    L = G.host_lib()
+    L.tch_copy_pool_run.restype = C.c_int
+    L.tch_copy_pool_run.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+    rng = np.random.default_rng(5)
+    sizes = [1, 255, 4096, (2 << 20) - 4, (2 << 20) + 12, 5 << 20, (3 << 20) + 7]
+    srcs = [rng.integers(0, 256, s, dtype=np.uint8) for s in sizes]
+    for threads, callers in ((7, 4), (0, 2), (3, 1)):
+        dsts = [[np.zeros(s, np.uint8) for s in sizes] for _ in range(callers)]
+        dp = (C.c_void_p * (callers * len(sizes)))(*[d.ctypes.data for row in dsts for d in row])
+        sp = (C.c_void_p * len(sizes))(*[a.ctypes.data for a in srcs])
+        bp = (C.c_int64 * len(sizes))(*sizes)
+        assert L.tch_copy_pool_run(threads, callers, 6, len(sizes), dp, sp, bp) == 0
+        for row in dsts:
+            for d, a in zip(row, srcs):
+                np.testing.assert_array_equal(d, a)
