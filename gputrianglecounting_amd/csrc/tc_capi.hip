@@ -320,6 +320,26 @@ void record(tc_ctx* c, int i, hipStream_t s) {
   if (c->timing && c->ev[i]) (void)hipEventRecord(c->ev[i], s);
 }
 
+// A single graph's oriented count in four launches: the counters cleared,
+// one prep kernel (widened offsets, degrees, huge-row list, piece rows, the
+// output cleared), the orientation, the count.  OffT/IT: the caller's offset
+// and id types; out nullable (the total only), total nullable.
+template <typename OffT, typename IT, typename OT>
+hipError_t oriented_single(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int64_t nv, int64_t nnz, const OffT* off,
+                           const IT* ind, OT* out, int64_t* total, hipStream_t s) {
+  hipError_t e = tcd::launch_fill_i64(p.dscal, 8, 0, s);
+  if (e != hipSuccess) return e;
+  tcd::OrientLaunch o = orient_launch(cfg, nv, nnz, false);
+  if (!out || sizeof(OT) != sizeof(int32_t)) o.window = tcd::kOrientWindow;  // other windows: int32 counts only
+  const int kind = !out ? 0 : sizeof(OT) == sizeof(int32_t) ? 1 : 2;
+  if ((e = tcd::launch_prep_single(p, nv, nnz, off, tcd::orient_piece_of(o, kind, c->cus, nnz), out, total, s)) !=
+      hipSuccess)
+    return e;
+  if ((e = tcd::launch_orient(p, nv, nnz, ind, c->cus, s, true)) != hipSuccess) return e;
+  record(c, 1, s);
+  return tcd::launch_count_orient(p, nv, nnz, out, total, o, c->cus, s, true);
+}
+
 // Persistent grid for the Merge Path kernel (every workgroup resident) and a
 // fixed grid for the one-lane-per-edge kernel.
 int count_wg(const tc_ctx* c, const tc_cfg& cfg, int vi, int which = 0) {
@@ -365,20 +385,17 @@ int count32(tc_ctx* c, void* base, const Layout& L, const tc_cfg& cfg, int vi, i
   int rc = TC_OK;
   tcd::Prep p = make_prep(base, L, false);
   record(c, 0, s);
-  if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
   if (cfg.oriented) {
     // Degree-ordered orientation: triangle {a, b, c} with (d, id) order
     // a < b < c is found once, at arc a -> b with match c; every one of the
     // three gets 2 (out[v] = 2 T(v)), the total 6 per triangle.
     // No edge list is built: tc_last_edge_work reports (0, 0) for this mode.
-    if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-    if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-    record(c, 1, s);
-    if ((e = count_oriented(c, p, cfg, nv, nnz, d_out, d_total, s)) != hipSuccess)
-      return hip_fail(c, e, "count kernel launch");
+    if ((e = oriented_single(c, p, cfg, nv, nnz, d_off, d_ind, d_out, d_total, s)) != hipSuccess)
+      return hip_fail(c, e, "oriented count launch");
   } else {
+    if ((e = tcd::launch_fill_i32(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
+    if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
     if ((e = tcd::launch_build_edges(p, nv, p.voff, d_ind, 0, c->cus, s)) != hipSuccess)
       return hip_fail(c, e, "edge list");
     record(c, 1, s);
@@ -665,13 +682,8 @@ int count_i64(tc_ctx* c, int64_t nv, int64_t nnz, const int64_t* d_off, const in
   tcd::Prep p = make_prep(c->pool, L, false);
   p.voff = const_cast<int64_t*>(d_off);  // the caller's offsets are already int64; the kernels only read them
   record(c, 0, s);
-  if (d_out && (e = tcd::launch_fill_i64(d_out, nv, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if (d_total && (e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-  record(c, 1, s);
-  if ((e = count_oriented(c, p, cfg, nv, nnz, d_out, d_total, s)) != hipSuccess)
-    return hip_fail(c, e, "count kernel launch");
+  if ((e = oriented_single(c, p, cfg, nv, nnz, d_off, d_ind, d_out, d_total, s)) != hipSuccess)
+    return hip_fail(c, e, "oriented count launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
   record(c, 3, s);
@@ -714,13 +726,9 @@ int tc_count_total_i32(tc_ctx* c, int32_t nv, int64_t nnz, const int32_t* d_off,
   // The degree-ordered orientation finds every triangle once; without
   // per-vertex output the count kernel keeps only the total (6 per
   // triangle).  No edge list: tc_last_edge_work reports (0, 0).
-  if ((e = tcd::launch_fill_i64(d_total, 1, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_fill_i64(p.dscal, 2, 0, s)) != hipSuccess) return hip_fail(c, e, "fill");
-  if ((e = tcd::launch_voff_from_off32(d_off, nv, p.voff, s)) != hipSuccess) return hip_fail(c, e, "voff");
-  if ((e = tcd::launch_orient(p, nv, nnz, d_ind, c->cus, s)) != hipSuccess) return hip_fail(c, e, "orient");
-  record(c, 1, s);
-  if ((e = count_oriented(c, p, cfg, nv, nnz, static_cast<int32_t*>(nullptr), d_total, s)) != hipSuccess)
-    return hip_fail(c, e, "count kernel launch");
+  if ((e = oriented_single(c, p, cfg, nv, nnz, d_off, d_ind, static_cast<int32_t*>(nullptr), d_total, s)) !=
+      hipSuccess)
+    return hip_fail(c, e, "oriented count launch");
   c->last_dscal = p.dscal;
   record(c, 2, s);
   record(c, 3, s);

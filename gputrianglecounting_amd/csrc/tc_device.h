@@ -122,9 +122,21 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
                               int cus, hipStream_t s, int64_t lim = INT64_MAX);
 // Orientation in place (k_orient_rows) into p.orow / p.oind of a CSR with
 // nnz arcs (the size p was laid out for); ind holds graph-local ids when
-// p.vbase is set.
-hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s);
-hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s);
+// p.vbase is set.  prepped: launch_prep_single has run (degrees and the
+// huge-row list are written, p.dscal[0..7] were zeroed before it).
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s,
+                         bool prepped = false);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s,
+                         bool prepped = false);
+// A single graph's prep in one launch: widened offsets (int32 off; an int64
+// CSR's offsets are p.voff itself), degrees, the huge-row list, the piece
+// rows for pieces of `piece` arcs (orient_piece_of) and the output cleared
+// (out nullable, total nullable).  p.dscal[0..7] must be zero; then
+// launch_orient and launch_count_orient run with prepped = true.
+hipError_t launch_prep_single(const Prep& p, int64_t nv, int64_t nnz, const int32_t* off, int64_t piece,
+                              int32_t* out, int64_t* total, hipStream_t s);
+hipError_t launch_prep_single(const Prep& p, int64_t nv, int64_t nnz, const int64_t* off, int64_t piece,
+                              int64_t* out, int64_t* total, hipStream_t s);
 // Persistent grid: as many workgroups as fit on the device at once.
 int resident_wgs(int variant_idx, int cus);
 // which = 1: the second kernel of a two-launch variant
@@ -170,10 +182,14 @@ struct OrientLaunch {
 int resident_orient_wgs(const OrientLaunch& o, int kind, int cus);
 // Per-vertex counts (out; nullptr: the total only) over the orientation in
 // p.orow / p.oind of nnz arcs.
+// prepped: launch_prep_single wrote the piece rows and zeroed the counter.
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total,
-                               const OrientLaunch& o, int cus, hipStream_t s);
+                               const OrientLaunch& o, int cus, hipStream_t s, bool prepped = false);
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total,
-                               const OrientLaunch& o, int cus, hipStream_t s);
+                               const OrientLaunch& o, int cus, hipStream_t s, bool prepped = false);
+// The piece size launch_count_orient takes for this launch shape (kind 0 =
+// the total only, 1 = int32 counts, 2 = int64 counts).
+int64_t orient_piece_of(const OrientLaunch& o, int kind, int cus, int64_t nnz);
 hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, const int32_t* vcnt, int32_t* out,
                               int64_t* total, hipStream_t s);
 hipError_t launch_validate(const int32_t* off, const int32_t* ind, int32_t nv, int64_t nnz, int32_t* flag,

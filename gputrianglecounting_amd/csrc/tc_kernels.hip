@@ -1112,6 +1112,47 @@ __global__ void k_piece_rows(const int64_t* __restrict__ toff, int64_t nv, int64
   }
 }
 
+// Everything a single graph needs before its orientation, in one pass over
+// the rows (in place of the output and total fills, k_voff_from_off32,
+// k_huge_rows and k_piece_rows): the widened offsets (int32 input; an
+// int64 CSR's offsets are read in place, voff == nullptr), every vertex's
+// degree, the rows of more than kHugeRow arcs listed (hcnt zeroed before),
+// the row of the first arc position of each piece of P, and the output
+// cleared (out[v] = 0, *total = 0).
+template <typename OffT, typename OT>
+__global__ __launch_bounds__(256) void k_prep_single(const OffT* __restrict__ off, int64_t nv,
+                                                     int64_t* __restrict__ voff, int32_t* __restrict__ deg,
+                                                     int32_t* __restrict__ huge, int64_t cap,
+                                                     unsigned long long* hcnt, int64_t P, int32_t* __restrict__ prow,
+                                                     OT* __restrict__ out, int64_t* __restrict__ total) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (total) *total = 0;
+    if (voff) voff[nv] = (int64_t)off[nv];
+  }
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nv; base += step) {  // uniform per workgroup
+    const int64_t u = base + threadIdx.x;
+    int64_t b = 0, e = 0;
+    if (u < nv) {
+      b = (int64_t)off[u];
+      e = (int64_t)off[u + 1];
+      if (voff) voff[u] = b;
+      deg[u] = (int32_t)(e - b);  // a simple graph's degree is below nv < 2^31
+      if (out) out[u] = 0;
+      for (int64_t m = (b + P - 1) / P; m * P < e; ++m) prow[m] = (int32_t)u;
+    }
+    const bool h = e - b > kHugeRow;
+    const uint64_t m = __ballot(h);
+    if (!m) continue;
+    unsigned long long at = 0;
+    if (lane == 0) at = atomicAdd(hcnt, (unsigned long long)__popcll(m));
+    const int64_t i = (int64_t)__shfl(at, 0, 64) + __popcll(m & lt);
+    if (h && i < cap) huge[i] = (int32_t)u;
+  }
+}
+
 // Next-chunk prefetch and waves per SIMD of the oriented count kernel (its
 // workgroup size is a launch argument, OrientLaunch).
 constexpr bool kOrientPF = true;
@@ -1237,18 +1278,19 @@ int resident_orient_wgs(const OrientLaunch& o, int kind, int cus) {
 
 template <bool PV, typename OT>
 hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out, int64_t* total,
-                                 const OrientLaunch& o, int kind, int cus, hipStream_t s) {
+                                 const OrientLaunch& o, int kind, int cus, hipStream_t s, bool prepped) {
   if (!orient_shape_ok(o, kind) || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
-  hipError_t e = launch_fill_i64(p.dscal + 4, 1, 0, s);
+  hipError_t e = prepped ? hipSuccess : launch_fill_i64(p.dscal + 4, 1, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
   const int nwg = resident_orient_wgs(o, kind, cus);
   const int wg = o.lookup ? 256 : o.wg;
   const int64_t piece = orient_piece(nnz, o.piece, (wg / 64) * (int64_t)nwg);
-  hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, piece,
-                     p.prow);
+  if (!prepped)
+    hipLaunchKernelGGL(k_piece_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
+                       piece, p.prow);
   // a wave for every piece: the kernel that takes one piece per wave
   const bool one = (wg / 64) * (int64_t)nwg * piece >= nnz;
   // rows longer than a wave's window are listed first; the table kernel
@@ -1290,14 +1332,43 @@ hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out
   return hipGetLastError();
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int32_t* out, int64_t* total,
-                               const OrientLaunch& o, int cus, hipStream_t s) {
-  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, o, 1, cus, s)
-             : launch_count_orient_t<false>(p, nv, nnz, out, total, o, 0, cus, s);
+                               const OrientLaunch& o, int cus, hipStream_t s, bool prepped) {
+  return out ? launch_count_orient_t<true>(p, nv, nnz, out, total, o, 1, cus, s, prepped)
+             : launch_count_orient_t<false>(p, nv, nnz, out, total, o, 0, cus, s, prepped);
 }
 hipError_t launch_count_orient(const Prep& p, int64_t nv, int64_t nnz, int64_t* out, int64_t* total,
-                               const OrientLaunch& o, int cus, hipStream_t s) {
-  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, o, 0, cus, s);
-  return launch_count_orient_t<true>(p, nv, nnz, out, total, o, 2, cus, s);
+                               const OrientLaunch& o, int cus, hipStream_t s, bool prepped) {
+  if (!out) return launch_count_orient_t<false>(p, nv, nnz, (int32_t*)nullptr, total, o, 0, cus, s, prepped);
+  return launch_count_orient_t<true>(p, nv, nnz, out, total, o, 2, cus, s, prepped);
+}
+
+int64_t orient_piece_of(const OrientLaunch& o, int kind, int cus, int64_t nnz) {
+  const int nwg = resident_orient_wgs(o, kind, cus);
+  const int wg = o.lookup ? 256 : o.wg;
+  return orient_piece(nnz, o.piece, (wg / 64) * (int64_t)nwg);
+}
+
+template <typename OffT, typename OT>
+hipError_t launch_prep_single_t(const Prep& p, int64_t nv, int64_t nnz, const OffT* off, int64_t piece, OT* out,
+                                int64_t* total, hipStream_t s) {
+  if (nv <= 0) {  // nothing to orient or count: the total alone
+    if (!total) return hipSuccess;
+    return launch_fill_i64(total, 1, 0, s);
+  }
+  if (!p.huge || !p.odeg || !p.prow || piece < 1) return hipErrorInvalidValue;
+  int64_t* voff = sizeof(OffT) == sizeof(int64_t) ? nullptr : p.voff;
+  hipLaunchKernelGGL((k_prep_single<OffT, OT>), dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, off, nv, voff,
+                     p.odeg, p.huge, nnz / kHugeRow + 1, reinterpret_cast<unsigned long long*>(p.dscal + 6), piece,
+                     p.prow, out, total);
+  return hipGetLastError();
+}
+hipError_t launch_prep_single(const Prep& p, int64_t nv, int64_t nnz, const int32_t* off, int64_t piece,
+                              int32_t* out, int64_t* total, hipStream_t s) {
+  return launch_prep_single_t(p, nv, nnz, off, piece, out, total, s);
+}
+hipError_t launch_prep_single(const Prep& p, int64_t nv, int64_t nnz, const int64_t* off, int64_t piece,
+                              int64_t* out, int64_t* total, hipStream_t s) {
+  return launch_prep_single_t(p, nv, nnz, off, piece, out, total, s);
 }
 
 hipError_t launch_mark_list(const int32_t* list, int64_t nlist, int64_t nv, int32_t* vmark, hipStream_t s) {
@@ -1314,14 +1385,17 @@ hipError_t launch_gather_list(const int32_t* list, int64_t nlist, int64_t nv, co
 }
 
 template <typename IT>
-hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind, int cus, hipStream_t s) {
+hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind, int cus, hipStream_t s,
+                           bool prepped) {
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
-  hipError_t e = launch_fill_i64(p.dscal + 5, 3, 0, s);  // task counter, huge rows listed and taken
+  // task counter, huge rows listed and taken
+  hipError_t e = prepped ? hipSuccess : launch_fill_i64(p.dscal + 5, 3, 0, s);
   if (e != hipSuccess || nv <= 0) return e;
   if (!p.huge || !p.odeg) return hipErrorInvalidValue;
   const int64_t cap = nnz / kHugeRow + 1;
-  hipLaunchKernelGGL(k_huge_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv, p.huge,
-                     cap, reinterpret_cast<unsigned long long*>(p.dscal + 6), p.odeg);
+  if (!prepped)
+    hipLaunchKernelGGL(k_huge_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
+                       p.huge, cap, reinterpret_cast<unsigned long long*>(p.dscal + 6), p.odeg);
   // at most one wave per 64 vertices: ceil(nv / 256) workgroups of 4 waves, capped
   hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
                      (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
@@ -1330,11 +1404,13 @@ hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind
                      reinterpret_cast<unsigned long long*>(p.dscal + 7), (const int32_t*)p.odeg);
   return hipGetLastError();
 }
-hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s) {
-  return launch_orient_t(p, nv, nnz, ind, cus, s);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s,
+                         bool prepped) {
+  return launch_orient_t(p, nv, nnz, ind, cus, s, prepped);
 }
-hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s) {
-  return launch_orient_t(p, nv, nnz, ind, cus, s);
+hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s,
+                         bool prepped) {
+  return launch_orient_t(p, nv, nnz, ind, cus, s, prepped);
 }
 
 int occupancy(void (*fn)(CountArgs), int bs) {
