@@ -50,9 +50,18 @@ __device__ __forceinline__ uint32_t cuckoo_mult(int i) {
   }
 }
 
-// Bucket of an id among nb buckets under multiplier c.
+// Bucket of an id among nb buckets (nb < 2^16) under multiplier c: the top
+// 24 bits x of the id times c (mod 2^32), scaled to nb as x * nb / 2^24.
+// Both products are full-rate 24-bit multiplies (a 32-bit multiply issues
+// at a quarter of the rate, and the lookups are bound by instruction issue);
+// ids of 2^24 and more are folded into the 24 bits the multiply reads.  The
+// scaling is the high word of a 24 x 24-bit product (x times nb * 2^8).
+// HIP's __umul24 returns int: its product is taken as unsigned before any
+// shift (a signed shift of it gave out-of-range buckets).
 __device__ __forceinline__ uint32_t hash_bucket(int32_t key, uint32_t c, uint32_t nb) {
-  return (uint32_t)(((uint64_t)((uint32_t)key * c) * nb) >> 32);
+  const uint32_t k = (uint32_t)key ^ ((uint32_t)key >> 24);
+  const uint32_t x = (uint32_t)__umul24(k, c) >> 8;
+  return (uint32_t)(((uint64_t)x * (uint64_t)((nb << 8) & 0xffffffu)) >> 32);
 }
 
 // The table of one window: K (2 nb slots), the hash pair (c1, c2), or
@@ -65,11 +74,13 @@ struct HashTab {
   bool sorted;
 };
 
-// Looks the valid elements of e[0..N) up in the table: a hit adds 2 to the
-// slot's counter (XC); returns the hits.
+// Looks the elements of e[0..N) up in the table (an element outside the
+// list holds kNoId, which equals no key and no empty slot): a hit in slot
+// 2b + h adds 2 to the slot's 16-bit counter, half h of counter word b
+// (XC); returns the hits.
+constexpr int32_t kNoId = -2;
 template <int N, bool XC>
-__device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const bool (&ok)[N], const HashTab& t,
-                                              int32_t* X) {
+__device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const HashTab& t, int32_t* X) {
   int32_t hits = 0;
   if (!t.sorted) {
     const int2* K2 = reinterpret_cast<const int2*>(t.K);
@@ -87,14 +98,14 @@ __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const bool 
     }
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-      uint32_t slot = 0xffffffffu;
-      if (k1[q].x == e[q]) slot = 2 * b1[q];
-      if (k1[q].y == e[q]) slot = 2 * b1[q] + 1;
-      if (k2[q].x == e[q]) slot = 2 * b2[q];
-      if (k2[q].y == e[q]) slot = 2 * b2[q] + 1;
-      if (ok[q] && slot != 0xffffffffu) {
+      const bool in1 = k1[q].x == e[q] || k1[q].y == e[q];
+      const bool hit = in1 || k2[q].x == e[q] || k2[q].y == e[q];
+      if (hit) {  // a key sits in one slot only
         ++hits;
-        if (XC) x_credit(X, (int32_t)slot);
+        if (XC) {
+          const bool upper = k1[q].y == e[q] || k2[q].y == e[q];
+          atomicAdd(&X[in1 ? b1[q] : b2[q]], upper ? 0x20000 : 2);
+        }
       }
     }
   } else {
@@ -105,7 +116,7 @@ __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const bool 
       int32_t pos = 0;
       for (int32_t st = t.top; st > 0; st >>= 1)
         if (pos + st <= t.n && t.K[pos + st - 1] < e[q]) pos += st;
-      if (ok[q] && pos < t.n && t.K[pos] == e[q]) {
+      if (pos < t.n && t.K[pos] == e[q]) {
         ++hits;
         if (XC) x_credit(X, pos);
       }
@@ -125,16 +136,18 @@ __device__ __forceinline__ int32_t hash_stream(const int32_t* __restrict__ bp, i
   const int32_t nvec = (int32_t)(((g1 + 3) >> 2) - (g0 >> 2));
   const int32_t lo = (int32_t)(g0 & 3), hi = (int32_t)(g1 - (g0 & ~int64_t(3)));  // valid element offsets
   int32_t cnt = 0;
-  int4 nx = make_int4(-1, -1, -1, -1);
+  int4 nx = make_int4(kNoId, kNoId, kNoId, kNoId);
   if (gl < nvec) nx = b4[gl];
   for (int32_t j = gl; j < nvec; j += LG) {
     const int4 a = nx;
     if (j + LG < nvec) nx = b4[j + LG];
-    const int32_t e[4] = {a.x, a.y, a.z, a.w};
-    bool ok[4];
+    int32_t e[4] = {a.x, a.y, a.z, a.w};
+    if (4 * j < lo || 4 * j + 4 > hi) {  // a list's first or last vector: elements outside it never match
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ok[q] = 4 * j + q >= lo && 4 * j + q < hi;
-    cnt += hash_probe<4, XC>(e, ok, t, X);
+      for (int q = 0; q < 4; ++q)
+        if ((uint32_t)(4 * j + q - lo) >= (uint32_t)(hi - lo)) e[q] = kNoId;
+    }
+    cnt += hash_probe<4, XC>(e, t, X);
   }
   return cnt;
 }
