@@ -128,8 +128,9 @@ int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch);
 int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false) {
   if (cfg.row_groups > 0) return cfg.row_groups;
   if (!batch && nnz >= (int64_t(1) << 30)) return 1;
-  // the table kernel streams one partner per wave (s24: 175.0 ms; 2: 199.8)
-  if (!batch && nnz >= (int64_t(1) << 28) && orient_lookup(cfg, nnz, batch) == 1) return 1;
+  // the table kernel: one partner per wave from 2^28 arcs (s24: 143.4 ms;
+  // 2: 144.8), two below (s22: 26.6 ms; 1: 27.7)
+  if (!batch && orient_lookup(cfg, nnz, batch) == 1) return nnz >= (int64_t(1) << 28) ? 1 : 2;
   if (!batch && nnz >= (int64_t(1) << 28)) return 2;
   const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
   if (dbar < 10) return 32;
@@ -167,12 +168,13 @@ int orient_side(const tc_cfg& cfg, bool batch, int64_t nnz) {
 // walked by each lane (k_count_orient), 1 = an LDS cuckoo table probed
 // twice per element of wave-coalesced reads (k_count_hash), 2 = the same
 // kernel searching a sorted window per element.  cfg.lookup 1, 2 and 3 ask
-// for them; 0 (automatic) takes the table for single graphs from 2^28 arcs
-// up and the walked array otherwise.  Measured count times, walked/table:
-// R-MAT s24 188.7/175.0 ms (both with long rows), s20 6.3/8.1 ms, RGG n=4M
-// 1.7/3.3 ms: the table's coalesced reads pay off on long partner lists,
-// its build on short rows does not.
-constexpr int64_t kTableArcs = int64_t(1) << 28;
+// for them; 0 (automatic) takes the table for single graphs from 3 * 2^25
+// arcs up and the walked array otherwise.  Measured count times, walked /
+// table (each with its best partners per wave): R-MAT s24 183.1/143.4 ms,
+// s22 (1.3e8 arcs) 34.5/26.5, s21 (6.4e7) 13.9/14.7, s20 6.3/6.6, RGG n=4M
+// 1.7/2.7, the mixed 4096-graph batch 32.5/40.5: the table's coalesced reads
+// pay off on long partner lists, its build on short rows does not.
+constexpr int64_t kTableArcs = int64_t(3) << 25;
 int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch) {
   if (cfg.lookup >= 1 && cfg.lookup <= 3) return cfg.lookup - 1;
   return !batch && nnz >= kTableArcs ? 1 : 0;

@@ -81,17 +81,18 @@ typedef struct tc_ctx tc_ctx;
  *   row_groups     oriented scheme: partners of a row a wave streams at once,
  *                  each by 64/row_groups lanes (1, 2, 4, 8, 16 or 32;
  *                  default -1 = chosen from the size and mean degree: 1
- *                  from 2^30 arcs up, from 2^28 (single graphs) 1 with the
- *                  table lookup and 2 with the walked array, else 32
- *                  below mean degree 10, 16 below 20, 8 above)
+ *                  from 2^30 arcs up; single graphs with the table lookup 1
+ *                  from 2^28 arcs and 2 below; with the walked array 2 from
+ *                  2^28 arcs; else 32 below mean degree 10, 16 below 20, 8
+ *                  above)
  *   lookup         oriented scheme: how a partner's list meets the row's
  *                  window: 1 = the window as a sorted LDS array that each
  *                  lane walks (k_count_orient); 2 = an LDS cuckoo table
  *                  that every element of wave-coalesced 16-byte reads
  *                  probes twice (k_count_hash); 3 = the same reads, each
  *                  element binary-searched in the sorted window;
- *                  0 (default) = automatic: 2 for single graphs from 2^28
- *                  arcs up (with row_groups 1), else 1
+ *                  0 (default) = automatic: 2 for single graphs from
+ *                  3 * 2^25 arcs up, else 1
  *   long_rows      oriented scheme: 2 = rows whose N+ is longer than one
  *                  wave's LDS window (`window` ids for lookup 1, 624 for 2
  *                  and 3) are counted by whole workgroups sharing one table of
