@@ -324,16 +324,19 @@ __device__ __forceinline__ int64_t long_rows_phase(int32_t* K, int32_t* X, int64
               const int32_t w = oind[tu + jj];
               const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
               c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
-#pragma unroll
-              for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+              c = group_sum32<LG>(c);  // G = 1: every lane is here (j is the same in all)
               if (PV && gl == 0 && c) out_add(out, w, 2 * c);
             }
             cg += c;
           }
-          int64_t rs = gl == 0 ? cg : 0;
+          if (G == 1) {
+            cu += uniform64(cg);  // every lane holds the same sums
+          } else {
+            int64_t rs = gl == 0 ? cg : 0;
 #pragma unroll
-          for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
-          cu += uniform64(rs);
+            for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
+            cu += uniform64(rs);
+          }
         } else {
           for (int64_t j = r0 + wv; j < r1; j += NW) {
             const int32_t w = oind[tu + j];
@@ -344,7 +347,7 @@ __device__ __forceinline__ int64_t long_rows_phase(int32_t* K, int32_t* X, int64
               g1 = wave_lower_bound(ind4, g0, g1, amax == INT_MAX ? amax : amax + 1, lane);
             }
             int32_t c = hash_stream<64, PV>(ind4, g0, g1, lane, tab, X);
-            c = wave_sum32(c);
+            c = wave_sum32_dpp(c);
             if (c) {
               if (PV && lane == 0) out_add(out, w, 2 * c);
               cu += c;
@@ -451,16 +454,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               const int32_t w = oind[tu + jj];
               const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
               c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
-#pragma unroll
-              for (int o = LG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+              c = group_sum32<LG>(c);  // G = 1: every lane is here (j is the same in all)
               if (PV && gl == 0 && c) out_add(out, w, 2 * c);
             }
             cg += c;
           }
-          int64_t rs = gl == 0 ? cg : 0;
+          if (G == 1) {
+            cu += uniform64(cg);  // every lane holds the same sums
+          } else {
+            int64_t rs = gl == 0 ? cg : 0;
 #pragma unroll
-          for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
-          cu += uniform64(rs);
+            for (int o = 32; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 64);
+            cu += uniform64(rs);
+          }
         } else if (WIN) {
           for (int64_t j = 0; j < lt; ++j) {
             const int32_t w = oind[tu + j];
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               g1 = wave_lower_bound(ind4, g0, g1, amax == INT_MAX ? amax : amax + 1, lane);
             }
             int32_t c = hash_stream<64, PV>(ind4, g0, g1, lane, tab, X);
-            c = wave_sum32(c);
+            c = wave_sum32_dpp(c);
             if (c) {
               if (PV && lane == 0) out_add(out, w, 2 * c);
               cu += c;

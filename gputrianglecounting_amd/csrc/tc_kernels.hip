@@ -42,6 +42,31 @@ __device__ __forceinline__ int32_t wave_sum32(int32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// The same sum by DPP: row shifts within rows of 16 lanes, then row
+// broadcasts (lane 15 into row 1 and 3, lane 31 into rows 2 and 3), six
+// adds with no LDS round trip; lane 63 ends with the total, read into every
+// lane.  Every lane of the wave must be active.
+__device__ __forceinline__ int32_t wave_sum32_dpp(int32_t v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+// Sum over aligned groups of LG lanes, every lane of a group getting its
+// group's sum; LG = 64 takes the DPP form (every lane active).
+template <int LG>
+__device__ __forceinline__ int32_t group_sum32(int32_t v) {
+  if constexpr (LG == 64) {
+    return wave_sum32_dpp(v);
+  } else {
+#pragma unroll
+    for (int o = LG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  }
+}
 
 // Block-wide sum; every thread gets the result.  red must hold BS/64 slots.
 template <int BS>
