@@ -51,16 +51,17 @@ __device__ __forceinline__ uint32_t cuckoo_mult(int i) {
 }
 
 // Bucket of an id among nb buckets (nb < 2^16) under multiplier c: the top
-// 24 bits x of the id times c (mod 2^32), scaled to nb as x * nb / 2^24.
-// Both products are full-rate 24-bit multiplies (a 32-bit multiply issues
-// at a quarter of the rate, and the lookups are bound by instruction issue);
-// ids of 2^24 and more are folded into the 24 bits the multiply reads.  The
-// scaling is the high word of a 24 x 24-bit product (x times nb * 2^8).
-// HIP's __umul24 returns int: its product is taken as unsigned before any
-// shift (a signed shift of it gave out-of-range buckets).
+// 24 bits x of the id's low 24 bits times c (mod 2^32), scaled to nb as
+// x * nb / 2^24.  Both products are full-rate 24-bit multiplies (a 32-bit
+// multiply issues at a quarter of the rate, and the lookups are bound by
+// instruction issue).  Ids of 2^24 and more share the buckets of their low
+// 24 bits: a window where that overfills a bucket pair fails its builds and
+// is laid out sorted instead, so counts never depend on it.  The scaling is
+// the high word of a 24 x 24-bit product (x times nb * 2^8).  HIP's __umul24
+// returns int: its product is taken as unsigned before any shift (a signed
+// shift of it gave out-of-range buckets).
 __device__ __forceinline__ uint32_t hash_bucket(int32_t key, uint32_t c, uint32_t nb) {
-  const uint32_t k = (uint32_t)key ^ ((uint32_t)key >> 24);
-  const uint32_t x = (uint32_t)__umul24(k, c) >> 8;
+  const uint32_t x = (uint32_t)__umul24((uint32_t)key, c) >> 8;
   return (uint32_t)(((uint64_t)x * (uint64_t)((nb << 8) & 0xffffffu)) >> 32);
 }
 
