@@ -65,12 +65,30 @@ __device__ __forceinline__ uint32_t hash_bucket(int32_t key, uint32_t c, uint32_
   return (uint32_t)(((uint64_t)x * (uint64_t)((nb << 8) & 0xffffffu)) >> 32);
 }
 
+// Bucket of an id among 2^(32 - shift) buckets: the top bits of its low
+// 24 bits times c (mod 2^32), one multiply and one shift (small tables).
+__device__ __forceinline__ uint32_t hash_bucket_p2(int32_t key, uint32_t c, uint32_t shift) {
+  return (uint32_t)__umul24((uint32_t)key, c) >> shift;
+}
+
+// Slots of the table of a window of n ids on at most cap slots: nb buckets
+// of two, nb >= (2n + 2) / 3 (load at most 3/4): a power of two from 32 up
+// when 2 nb fits cap (buckets from one multiply and a shift), else the next
+// multiple of 32.
+__device__ __forceinline__ uint32_t table_slots(int32_t n, uint32_t cap) {
+  const uint32_t need = (uint32_t)((2 * n + 2) / 3);
+  uint32_t p2 = 32;
+  while (p2 < need) p2 *= 2;
+  return 2 * p2 <= cap ? 2 * p2 : 2 * ((need + 31) / 32 * 32);
+}
+
 // The table of one window: K (2 nb slots), the hash pair (c1, c2), or
 // sorted (n ids ascending in K[0 .. n), searched in `top` halving steps,
-// top = the largest power of two <= n).
+// top = the largest power of two <= n).  shift > 0: nb = 2^(32 - shift) and
+// buckets come from hash_bucket_p2.
 struct HashTab {
   const int32_t* K;
-  uint32_t nb, c1, c2;
+  uint32_t nb, c1, c2, shift;
   int32_t n, top;
   bool sorted;
 };
@@ -80,7 +98,7 @@ struct HashTab {
 // 2b + h adds 2 to the slot's 16-bit counter, half h of counter word b
 // (XC); returns the hits.
 constexpr int32_t kNoId = -2;
-template <int N, bool XC>
+template <int N, bool XC, bool P2>
 __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const HashTab& t, int32_t* X) {
   int32_t hits = 0;
   if (!t.sorted) {
@@ -89,8 +107,8 @@ __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const HashT
     int2 k1[N], k2[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-      b1[q] = hash_bucket(e[q], t.c1, t.nb);
-      b2[q] = hash_bucket(e[q], t.c2, t.nb);
+      b1[q] = P2 ? hash_bucket_p2(e[q], t.c1, t.shift) : hash_bucket(e[q], t.c1, t.nb);
+      b2[q] = P2 ? hash_bucket_p2(e[q], t.c2, t.shift) : hash_bucket(e[q], t.c2, t.nb);
     }
 #pragma unroll
     for (int q = 0; q < N; ++q) {
@@ -127,15 +145,13 @@ __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const HashT
 }
 
 // The elements at positions [g0, g1) of bp (16-byte aligned) looked up in
-// the table by lane gl of a group of LG lanes: vectors v0 + gl,
-// v0 + gl + LG, ..., the next one in flight while the current one is probed.
-template <int LG, bool XC>
-__device__ __forceinline__ int32_t hash_stream(const int32_t* __restrict__ bp, int64_t g0, int64_t g1, int gl,
-                                               const HashTab& t, int32_t* X) {
-  if (g0 >= g1) return 0;
-  const int4* b4 = reinterpret_cast<const int4*>(bp) + (g0 >> 2);
-  const int32_t nvec = (int32_t)(((g1 + 3) >> 2) - (g0 >> 2));
-  const int32_t lo = (int32_t)(g0 & 3), hi = (int32_t)(g1 - (g0 & ~int64_t(3)));  // valid element offsets
+// the table by lane gl of a group of LG lanes: vectors gl, gl + LG, ...,
+// the next one in flight while the current one is probed.  Only a list's
+// first and last vectors hold elements of other lists, which are set to
+// kNoId.
+template <int LG, bool XC, bool P2>
+__device__ __forceinline__ int32_t hash_stream_t(const int4* __restrict__ b4, int32_t nvec, int32_t lo, int32_t hi,
+                                                 int gl, const HashTab& t, int32_t* X) {
   int32_t cnt = 0;
   int4 nx = make_int4(kNoId, kNoId, kNoId, kNoId);
   if (gl < nvec) nx = b4[gl];
@@ -143,14 +159,25 @@ __device__ __forceinline__ int32_t hash_stream(const int32_t* __restrict__ bp, i
     const int4 a = nx;
     if (j + LG < nvec) nx = b4[j + LG];
     int32_t e[4] = {a.x, a.y, a.z, a.w};
-    if (4 * j < lo || 4 * j + 4 > hi) {  // a list's first or last vector: elements outside it never match
+    if ((uint32_t)(j - 1) >= (uint32_t)(nvec - 2)) {  // j == 0 or j == nvec - 1
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if ((uint32_t)(4 * j + q - lo) >= (uint32_t)(hi - lo)) e[q] = kNoId;
     }
-    cnt += hash_probe<4, XC>(e, t, X);
+    cnt += hash_probe<4, XC, P2>(e, t, X);
   }
   return cnt;
+}
+
+template <int LG, bool XC>
+__device__ __forceinline__ int32_t hash_stream(const int32_t* __restrict__ bp, int64_t g0, int64_t g1, int gl,
+                                               const HashTab& t, int32_t* X) {
+  if (g0 >= g1) return 0;
+  const int4* b4 = reinterpret_cast<const int4*>(bp) + (g0 >> 2);
+  const int32_t nvec = (int32_t)(((g1 + 3) >> 2) - (g0 >> 2));
+  const int32_t lo = (int32_t)(g0 & 3), hi = (int32_t)(g1 - (g0 & ~int64_t(3)));  // valid element offsets
+  if (t.shift) return hash_stream_t<LG, XC, true>(b4, nvec, lo, hi, gl, t, X);
+  return hash_stream_t<LG, XC, false>(b4, nvec, lo, hi, gl, t, X);
 }
 
 // Builds the table of the window src[0 .. n) (ascending ids) in K, tsz
@@ -162,10 +189,14 @@ __device__ __forceinline__ HashTab hash_build(const int32_t* __restrict__ src, i
   HashTab t;
   t.K = K;
   t.nb = tsz / 2;
+  t.shift = (t.nb & (t.nb - 1)) == 0 ? 32u - (uint32_t)__builtin_ctz(t.nb) : 0u;
   t.n = n;
   t.top = 0;
   t.sorted = true;
   t.c1 = t.c2 = 0;
+  auto bucket = [&](int32_t key, uint32_t c) {
+    return t.shift ? hash_bucket_p2(key, c, t.shift) : hash_bucket(key, c, t.nb);
+  };
   for (int r = 0; r < kCuckooSeeds && !sorted; ++r) {
     const uint32_t c1 = cuckoo_mult(2 * r), c2 = cuckoo_mult(2 * r + 1);
     for (uint32_t y = lane; y < tsz; y += 64) K[y] = -1;
@@ -175,11 +206,11 @@ __device__ __forceinline__ HashTab hash_build(const int32_t* __restrict__ src, i
     bool fail = false;
     for (int32_t y = lane; y < n; y += 64) {
       int32_t cur = src[y];
-      uint32_t b = hash_bucket(cur, c1, t.nb);  // the bucket cur is tried in first
+      uint32_t b = bucket(cur, c1);  // the bucket cur is tried in first
       for (int it = 0;; ++it) {
         if (atomicCAS(&K[2 * b], -1, cur) == -1 || atomicCAS(&K[2 * b + 1], -1, cur) == -1) break;
-        const uint32_t h1 = hash_bucket(cur, c1, t.nb);
-        const uint32_t o = b == h1 ? hash_bucket(cur, c2, t.nb) : h1;  // cur's other bucket
+        const uint32_t h1 = bucket(cur, c1);
+        const uint32_t o = b == h1 ? bucket(cur, c2) : h1;  // cur's other bucket
         if (atomicCAS(&K[2 * o], -1, cur) == -1 || atomicCAS(&K[2 * o + 1], -1, cur) == -1) break;
         if (it == kCuckooSteps) {
           fail = true;
@@ -219,10 +250,14 @@ __device__ __forceinline__ HashTab hash_build_wg(const int32_t* __restrict__ src
   HashTab t;
   t.K = K;
   t.nb = tsz / 2;
+  t.shift = (t.nb & (t.nb - 1)) == 0 ? 32u - (uint32_t)__builtin_ctz(t.nb) : 0u;
   t.n = n;
   t.top = 0;
   t.sorted = true;
   t.c1 = t.c2 = 0;
+  auto bucket = [&](int32_t key, uint32_t c) {
+    return t.shift ? hash_bucket_p2(key, c, t.shift) : hash_bucket(key, c, t.nb);
+  };
   for (int r = 0; r < kCuckooSeeds; ++r) {
     const uint32_t c1 = cuckoo_mult(2 * r), c2 = cuckoo_mult(2 * r + 1);
     for (uint32_t y = threadIdx.x; y < tsz; y += blockDim.x) K[y] = -1;
@@ -230,11 +265,11 @@ __device__ __forceinline__ HashTab hash_build_wg(const int32_t* __restrict__ src
     int fail = 0;
     for (int32_t y = threadIdx.x; y < n; y += blockDim.x) {
       int32_t cur = src[y];
-      uint32_t b = hash_bucket(cur, c1, t.nb);
+      uint32_t b = bucket(cur, c1);
       for (int it = 0;; ++it) {
         if (atomicCAS(&K[2 * b], -1, cur) == -1 || atomicCAS(&K[2 * b + 1], -1, cur) == -1) break;
-        const uint32_t h1 = hash_bucket(cur, c1, t.nb);
-        const uint32_t o = b == h1 ? hash_bucket(cur, c2, t.nb) : h1;
+        const uint32_t h1 = bucket(cur, c1);
+        const uint32_t o = b == h1 ? bucket(cur, c2) : h1;
         if (atomicCAS(&K[2 * o], -1, cur) == -1 || atomicCAS(&K[2 * o + 1], -1, cur) == -1) break;
         if (it == kCuckooSteps) {
           fail = 1;
@@ -302,7 +337,7 @@ __device__ __forceinline__ int64_t long_rows_phase(int32_t* K, int32_t* X, int64
     int64_t cu = 0;  // this wave's matches of the row
     for (int64_t ws = 0; ws < lu; ws += AWW) {
       const int32_t n = (int32_t)(lu - ws < AWW ? lu - ws : AWW);
-      const uint32_t tsz = 2u * (uint32_t)(((2 * n + 2) / 3 + 31) / 32 * 32);
+      const uint32_t tsz = table_slots(n, TSW);
       const HashTab tab = hash_build_wg(oind + ou + ws, n, K, tsz);
       const bool whole = n == lu;
       const int32_t amin = oind[ou + ws], amax = oind[ou + ws + n - 1];
@@ -437,8 +472,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       int64_t cu = 0;
       for (int64_t ws = 0; ws < lu; ws += AW) {
         const int32_t n = (int32_t)(lu - ws < AW ? lu - ws : AW);
-        // nb >= 2n/3 buckets of two slots (load <= 3/4), a multiple of 32
-        const uint32_t tsz = 2u * (uint32_t)(((2 * n + 2) / 3 + 31) / 32 * 32);
+        // nb >= 2n/3 buckets of two slots (load <= 3/4), a power of two when it fits
+        const uint32_t tsz = table_slots(n, TS);
         if (PV)
           for (uint32_t y = lane; y < tsz / 2; y += 64) X[y] = 0;
         const HashTab tab = hash_build(oind + ou + ws, n, K, tsz, sorted != 0, lane);
