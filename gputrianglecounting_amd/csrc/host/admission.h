@@ -88,9 +88,10 @@ class Arena {
 // oriented scheme (tc_capi.hip's layout): 32 bytes per vertex (widened
 // offsets, graph bases, N+ bounds, degrees), 4 per arc (the orientation)
 // and fewer than 1/32 per arc more (piece, hub-row and long-row lists), and
-// a fixed part (the piece list's slack of 65,536 entries, counters, the
-// rounding of ten regions) within kScratchFixed.
-constexpr size_t kScratchFixed = 1040 * kArenaAlign;
+// a fixed part (the piece list's slack of 65,536 entries, the scalars and
+// ticket counters, 4,352 bytes, the rounding of ten regions) within
+// kScratchFixed.
+constexpr size_t kScratchFixed = 1060 * kArenaAlign;
 inline size_t scratch_share(int64_t nv, int64_t nnz) {
   return 32 * (size_t)nv + 4 * (size_t)nnz + ((size_t)nnz + 31) / 32;
 }

@@ -219,7 +219,7 @@ Layout layout(int64_t nv, int64_t nnz, bool batch, Scheme sc) {
   L.ecap = edges ? (sc == Scheme::kListShadow ? 2 * nnz + 1 : list ? nnz + 1 : nnz / 2 + 1) : 0;
   L.voff = take(8 * (size_t)(nv + 1));
   L.vbase = take(batch ? 4 * (size_t)(nv + 1) : 0);
-  L.dscal = take(8 * 8);
+  L.dscal = take(8 * tcd::kScalars);
   L.qctr = take(4 * 24);
   L.ecnt = take(edges ? 4 * (size_t)(nv + 1) : 0);
   L.eoff = take(edges ? 8 * (size_t)(nv + 1) : 0);
@@ -277,6 +277,7 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
   p.ework = reinterpret_cast<int32_t*>(b + L.ework);
   p.wpre = reinterpret_cast<int64_t*>(b + L.wpre);
   p.dscal = reinterpret_cast<int64_t*>(b + L.dscal);
+  p.otask = reinterpret_cast<unsigned long long*>(p.dscal + tcd::kOtaskAt);
   p.qctr = reinterpret_cast<unsigned int*>(b + L.qctr);
   p.scan_tmp = reinterpret_cast<int64_t*>(b + L.scan);
   p.ecap = L.ecap;
@@ -329,7 +330,7 @@ void record(tc_ctx* c, int i, hipStream_t s) {
 template <typename OffT, typename IT, typename OT>
 hipError_t oriented_single(tc_ctx* c, const tcd::Prep& p, const tc_cfg& cfg, int64_t nv, int64_t nnz, const OffT* off,
                            const IT* ind, OT* out, int64_t* total, hipStream_t s) {
-  hipError_t e = tcd::launch_fill_i64(p.dscal, 8, 0, s);
+  hipError_t e = tcd::launch_fill_i64(p.dscal, tcd::kScalars, 0, s);
   if (e != hipSuccess) return e;
   tcd::OrientLaunch o = orient_launch(cfg, nv, nnz, false);
   if (!out || sizeof(OT) != sizeof(int32_t)) o.window = tcd::kOrientWindow;  // other windows: int32 counts only

@@ -74,6 +74,9 @@ inline int64_t orient_piece(int64_t nnz, int64_t base, int64_t waves) {
 // Rows longer than this are listed and oriented first, one row per wave, so
 // that a task of 64 vertices never holds several hub rows.
 constexpr int64_t kHugeRow = 4096;
+// The scalar block: dscal[0..7], then 16 ticket counters from dscal + kOtaskAt.
+constexpr int kOtaskAt = 32, kOtaskStride = 32;
+constexpr int kScalars = kOtaskAt + 16 * kOtaskStride;
 
 // Scratch sizing and prep/count launchers (stream-ordered, no host sync).
 struct Prep {
@@ -87,8 +90,12 @@ struct Prep {
   int32_t* ew;      // [ecap]
   int32_t* ework;   // [ecap]
   int64_t* wpre;    // [ecap + 1]
-  int64_t* dscal;   // [8]: E, W, overflow flag, -, piece counter, orientation task counter,
-                    // huge rows listed, huge rows taken
+  int64_t* dscal;   // [kScalars]: E, W, overflow flag, -, the edge build's vertex counter, -,
+                    // huge rows listed, huge rows taken; then the ticket counters (otask)
+  // 16 ticket counters kOtaskStride int64 apart (otask = dscal + kOtaskAt),
+  // so that no two share a 256-byte block: 0-7 hand out the orientation's
+  // tasks, 8-15 the count's pieces, ticket n of counter x being item 8 n + x
+  unsigned long long* otask;
   unsigned int* qctr;  // [24]: three launches' per-XCD range queues
   int64_t* scan_tmp;
   int64_t ecap;
@@ -123,7 +130,7 @@ hipError_t launch_build_edges(const Prep& p, int64_t nv, const int64_t* voff, co
 // Orientation in place (k_orient_rows) into p.orow / p.oind of a CSR with
 // nnz arcs (the size p was laid out for); ind holds graph-local ids when
 // p.vbase is set.  prepped: launch_prep_single has run (degrees and the
-// huge-row list are written, p.dscal[0..7] were zeroed before it).
+// huge-row list are written, p.dscal[0..kScalars) were zeroed before it).
 hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int32_t* ind, int cus, hipStream_t s,
                          bool prepped = false);
 hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* ind, int cus, hipStream_t s,
@@ -131,7 +138,7 @@ hipError_t launch_orient(const Prep& p, int64_t nv, int64_t nnz, const int64_t* 
 // A single graph's prep in one launch: widened offsets (int32 off; an int64
 // CSR's offsets are p.voff itself), degrees, the huge-row list, the piece
 // rows for pieces of `piece` arcs (orient_piece_of) and the output cleared
-// (out nullable, total nullable).  p.dscal[0..7] must be zero; then
+// (out nullable, total nullable).  p.dscal[0..kScalars) must be zero; then
 // launch_orient and launch_count_orient run with prepped = true.
 hipError_t launch_prep_single(const Prep& p, int64_t nv, int64_t nnz, const int32_t* off, int64_t piece,
                               int32_t* out, int64_t* total, hipStream_t s);

@@ -421,7 +421,8 @@ template <int TS, int AW, int G, int WPE, bool PV, typename OT, bool ONE, bool W
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_hash(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
-    unsigned long long* ctr, int side, int32_t P, int sorted, int64_t lmax, const int32_t* __restrict__ lrow,
+    unsigned long long* ctr, unsigned long long* pq, int side, int32_t P, int sorted, int64_t lmax,
+    const int32_t* __restrict__ lrow,
     const unsigned long long* nlong, int64_t lcap, unsigned long long* lctr) {
   static_assert(TS % 64 == 0 && 2 * (((2 * AW + 2) / 3 + 31) / 32 * 32) <= TS, "table of whole wave strides");
   constexpr int NW = 4;
@@ -445,13 +446,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     tri += long_rows_phase<NW * TS, GL, PV, OT>(keys_all, xcnt_all, &bcast, voff, orow, oind, lrow, nl, out, lctr,
                                                 by_in);
   }
+  const int64_t npieces = (M + P - 1) / P;
+  const int xcd = blockIdx.x & 7;
+  int qk = 0;
   for (;;) {
     int64_t b = 0;
     if (ONE) {
       b = (int64_t)blockIdx.x * NW + wv;
     } else {
-      if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
-      b = uniform64(__shfl(b, 0, 64));
+      b = next_task(pq, npieces, xcd, qk);
+      if (b < 0) break;
     }
     const int64_t a0 = b * P;
     if (a0 >= M) break;

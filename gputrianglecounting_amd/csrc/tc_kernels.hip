@@ -357,9 +357,10 @@ __device__ __forceinline__ bool orient_keep(int64_t dv, int64_t v, int64_t dw, i
 // scan, and a row never spills into another whatever the input (a CSR that
 // is not symmetric only orients differently).
 //
-// Waves take 64 vertices at a time from ctr (the grid has at most one wave
-// per 64 vertices, so a small graph does not crowd the counter with idle
-// waves).  A row of at most kShortRow arcs is done by its own lane, four
+// Waves take tasks of 64 vertices from eight counters, one per XCD (the
+// grid has at most one wave per 64 vertices, so a small graph does not
+// crowd the counters with idle waves).  A row of at most kShortRow arcs is
+// done by its own lane, four
 // arcs per step (sparse graphs have rows of a few arcs, which would leave
 // most of a wave idle); the longer rows of the 64 are then done one at a
 // time by the whole wave, 512 arcs per step with the next step's ids
@@ -435,15 +436,78 @@ __global__ void k_huge_rows(const int64_t* __restrict__ voff, int64_t nv, int32_
   }
 }
 
+// Inclusive max over the wave's lanes by DPP row shifts and broadcasts
+// (values >= 0; every lane active).
+__device__ __forceinline__ int32_t wave_incl_max_dpp(int32_t v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t x, int l) {
+  return (int64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(uint64_t)x, l) |
+         ((int64_t)__builtin_amdgcn_readlane((int32_t)((uint64_t)x >> 32), l) << 32);
+}
+
+// LDS writes of one wave made visible to its own later LDS reads (and kept
+// in order by the compiler).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows of a task of up to kWalkRow arcs are oriented together, as one arc
+// range walked in chunks of 64 consecutive arcs (one per lane).
+constexpr int64_t kWalkRow = 1024;
+// The next task (64 vertices from 64 t) for a wave of a workgroup on XCD
+// xcd, or -1 when every task is taken.  Task t belongs to counter t & 7
+// (otask[x * kOtaskStride]), whose n-th ticket is task 8 n + x; a wave takes
+// tickets from its own XCD's counter, then from the others in turn, moving
+// on once a counter has run past the last task (k counts the counters
+// left behind).  Every lane returns the same value.
+__device__ __forceinline__ int64_t next_task(unsigned long long* otask, int64_t T, int xcd, int& k) {
+  const int lane = threadIdx.x & 63;
+  for (; k < 8; ++k) {
+    const int x = (xcd + k) & 7;
+    if (x >= T) continue;  // this counter holds no task
+    int64_t g = 0;
+    if (lane == 0) g = (int64_t)atomicAdd(otask + (int64_t)x * kOtaskStride, 1ull);
+    const int64_t t = uniform64(__shfl(g, 0, 64)) * 8 + x;
+    if (t < T) return t;
+  }
+  return -1;
+}
+
 // IT: the caller's id type (int32 or int64 CSR); ids are below 2^31.
 // First the listed huge rows, one per wave from hctr (they start together
-// and run beside everything else), then tasks of 64 vertices from ctr,
-// whose long-row pass skips the listed rows.
-template <typename IT>
+// and run beside everything else), then tasks of 64 vertices from the 8
+// counters otask (next_task), whose long-row pass skips the listed rows.
+// One counter for every task serialised its atomics, about 9 ns each: 0.5 ms
+// of the 1.1 ms orientation of a 4M-vertex geometric graph (65,536 tasks).
+// Counters owning contiguous eighths of the vertex range instead of
+// interleaved tasks left an R-MAT graph's heavy low-id tasks to one XCD.
+//
+// WALK (the default): a task's rows of at most kWalkRow arcs are one arc
+// range [B, E) of the CSR, walked in chunks of 64 consecutive arcs, lane p
+// taking arc c0 + p, so every id load covers whole lines.  The owner lane of
+// each row that starts in the chunk marks the row's first position; an
+// inclusive max-scan over the marks gives each lane its row; each row's
+// kept arcs go to the front of its row and the others to the back, ranked
+// by ballots within the row's run of lanes, from front and back cursors
+// kept in LDS per row; the last lane of a row's run advances them.  Rows
+// longer than kWalkRow (and the listed huge rows) are stepped over and
+// oriented one at a time by the whole wave.  Without WALK, each row of at
+// most kShortRow arcs is oriented by its own lane.
+template <typename IT, bool WALK>
 __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__ voff, const IT* __restrict__ ind,
                                                      const int32_t* __restrict__ vbase, int64_t nv,
                                                      int32_t* __restrict__ oind, int64_t* __restrict__ orow,
-                                                     unsigned long long* ctr, const int32_t* __restrict__ huge,
+                                                     unsigned long long* otask, const int32_t* __restrict__ huge,
                                                      int64_t cap, const unsigned long long* nhuge,
                                                      unsigned long long* hctr, const int32_t* __restrict__ deg) {
   const int lane = threadIdx.x & 63;
@@ -462,11 +526,16 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       orient_row_wave(deg, ind, oind, orow, v, voff[v], voff[v + 1], vbase ? vbase[v] : 0, lane, lt);
     }
   }
+  __shared__ int4 s_wrow[WALK ? 256 : 1];   // per row of a task: start - B, degree, local id, graph base
+  __shared__ int2 s_wcur[WALK ? 256 : 1];   // per row: kept arcs placed, others placed
+  __shared__ int32_t s_wst[WALK ? 256 : 1];  // per chunk position: 1 + the lane whose row starts there
+  const int64_t T = (nv + 63) >> 6;
+  const int xcd = blockIdx.x & 7;
+  int k = 0;
   for (;;) {
-    int64_t v0 = 0;
-    if (lane == 0) v0 = (int64_t)atomicAdd(ctr, 64ull);
-    v0 = uniform64(__shfl(v0, 0, 64));
-    if (v0 >= nv) break;
+    const int64_t t = next_task(otask, T, xcd, k);
+    if (t < 0) break;
+    const int64_t v0 = t * 64;
     const int64_t v = v0 + lane;
     const bool mine = v < nv;
     int64_t b = 0, e = 0;
@@ -476,7 +545,71 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       e = voff[v + 1];
       vb = vbase ? vbase[v] : 0;
     }
-    if (mine && e - b <= kShortRow) {
+    if (WALK) {
+      int4* wrow = s_wrow + (threadIdx.x & ~63);
+      int2* wcur = s_wcur + (threadIdx.x & ~63);
+      int32_t* wst = s_wst + (threadIdx.x & ~63);
+      const int64_t d = e - b;
+      const bool walk = mine && d <= kWalkRow;
+      const bool obst = mine && d > kWalkRow;  // listed (done) or long (done below)
+      const int64_t B = uniform64(__shfl(b, 0, 64));  // lane 0 is always mine
+      const int last = __popcll(__ballot(mine)) - 1;
+      const int64_t E = readlane64(e, last);
+      wrow[lane] = make_int4((int32_t)(b - B), (int32_t)d, (int32_t)(v - vb), vb);
+      wcur[lane] = make_int2(0, 0);
+      int64_t c0 = B;
+      int32_t carry = 0;  // 1 + the lane whose row holds arc c0 when it started before c0
+      while (c0 < E) {
+        const uint64_t om = __ballot(obst && e > c0);  // the next row not walked
+        int64_t os = INT64_MAX, oe = 0;
+        if (om) {
+          const int l = __builtin_ctzll(om);
+          os = readlane64(b, l);
+          oe = readlane64(e, l);
+        }
+        if (os <= c0) {  // step over it
+          c0 = oe;
+          carry = 0;
+          continue;
+        }
+        int64_t cend = c0 + 64 < E ? c0 + 64 : E;
+        if (os < cend) cend = os;
+        const int32_t n = (int32_t)(cend - c0);  // 1 .. 64 arcs
+        const int32_t c0r = (int32_t)(c0 - B);
+        const bool valid = lane < n;
+        const int32_t w = valid ? (int32_t)ind[c0 + lane] : 0;
+        wst[lane] = lane == 0 ? carry : 0;
+        wave_lds_sync();
+        const int32_t rel = (int32_t)(b - B) - c0r;  // this lane's row start within the chunk
+        if (walk && d > 0 && rel >= 0 && rel < n) wst[rel] = lane + 1;
+        wave_lds_sync();
+        const int32_t r1 = wave_incl_max_dpp(wst[lane]);  // 1 + the lane whose row holds arc c0 + lane
+        const int r = r1 > 0 ? r1 - 1 : 0;
+        const int4 ri = wrow[r];
+        const int2 cur = wcur[r];
+        const int32_t dw = valid ? deg[(int64_t)ri.w + w] : 0;
+        const bool keep = valid && orient_keep(ri.y, ri.z, dw, w);
+        const bool other = valid && !keep;
+        const uint64_t mk = __ballot(keep), mi = __ballot(other);
+        const int32_t sg = ri.x - c0r > 0 ? ri.x - c0r : 0;  // first lane of this row's run
+        const uint64_t run = lt & ~((1ull << sg) - 1);       // lanes of the run before this one
+        const int32_t rk = __popcll(mk & run), rn = __popcll(mi & run);
+        const int64_t rowb = B + ri.x;
+        if (keep) oind[rowb + cur.x + rk] = ri.w + w;
+        if (other) oind[rowb + ri.y - 1 - cur.y - rn] = ri.w + w;
+        const int32_t se = ri.x + ri.y - c0r < n ? ri.x + ri.y - c0r : n;  // end of this row's run
+        wave_lds_sync();  // every lane has read its row's cursors
+        if (valid && lane == se - 1) wcur[r] = make_int2(cur.x + rk + (keep ? 1 : 0), cur.y + rn + (other ? 1 : 0));
+        wave_lds_sync();
+        carry = __builtin_amdgcn_readlane(r1, n - 1);
+        c0 = cend;
+      }
+      if (walk) {
+        orow[2 * v] = b;
+        orow[2 * v + 1] = b + wcur[lane].x;
+      }
+    }
+    if (!WALK && mine && e - b <= kShortRow) {
       const int64_t dv = e - b;
       const int32_t vl = (int32_t)(v - vb);
       int64_t pf = b, pb = e;
@@ -502,7 +635,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       orow[2 * v + 1] = pf;
     }
     // the long rows among the 64, one at a time by the whole wave
-    uint64_t lm = __ballot(mine && e - b > kShortRow && e - b <= kHugeRow);
+    uint64_t lm = __ballot(mine && e - b > (WALK ? kWalkRow : kShortRow) && e - b <= kHugeRow);
     while (lm) {
       const int l = __builtin_ctzll(lm);
       lm &= lm - 1;
@@ -909,10 +1042,10 @@ __global__ __launch_bounds__(256) void k_validate_batch(const int64_t* __restric
       else hi = mid;
     }
     const int64_t vb = tab[2 * lo], ab = tab[2 * lo + 1];
-    const int64_t ng = tab[2 * lo + 2] - vb, mg = tab[2 * lo + 3] - ab;
+    const int64_t ng = tab[2 * lo + 2] - vb;
     const int32_t* off = off_cat + vb + lo;  // graph-local offsets, ng + 1 of them
-    const int32_t* ind = ind_cat + ab;       // graph-local ids, mg of them
-    const int64_t k = K - ab;                // 0 <= k < mg
+    const int32_t* ind = ind_cat + ab;       // graph-local ids, tab[2 lo + 3] - ab of them
+    const int64_t k = K - ab;                // 0 <= k < the graph's arc count
     int64_t x = 0, xh = ng;                  // row of k: the last x with off[x] <= k
     while (xh - x > 1) {
       const int64_t mid = (x + xh) >> 1;
@@ -1184,7 +1317,7 @@ constexpr bool kOrientPF = true;
 constexpr int kOrientWPE = 8;
 template <typename OT>
 using OrientFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
-                          unsigned long long*, unsigned long long*, int, int32_t, int64_t);
+                          unsigned long long*, unsigned long long*, unsigned long long*, int, int32_t, int64_t);
 template <int AW, bool PV, typename OT, bool ONE>
 OrientFn<OT> orient_fn_w(int groups) {
   switch (groups) {
@@ -1218,7 +1351,8 @@ constexpr int kHashWPE = 8;
 constexpr int kHashLong = kHashAW;
 template <typename OT>
 using HashFn = void (*)(const int64_t*, const int64_t*, const int32_t*, const int32_t*, int64_t, OT*,
-                        unsigned long long*, unsigned long long*, int, int32_t, int, int64_t, const int32_t*,
+                        unsigned long long*, unsigned long long*, unsigned long long*, int, int32_t, int, int64_t,
+                        const int32_t*,
                         const unsigned long long*, int64_t, unsigned long long*);
 // WIN: rows longer than the window are walked in windows (without it the
 // kernel counts them first, in its long-row phase)
@@ -1306,7 +1440,8 @@ hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out
                                  const OrientLaunch& o, int kind, int cus, hipStream_t s, bool prepped) {
   if (!orient_shape_ok(o, kind) || nnz < 0) return hipErrorInvalidValue;
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(p.dscal + 4);
-  hipError_t e = prepped ? hipSuccess : launch_fill_i64(p.dscal + 4, 1, 0, s);
+  unsigned long long* pq = p.otask + 8 * kOtaskStride;  // the piece counters
+  hipError_t e = prepped ? hipSuccess : launch_fill_i64(reinterpret_cast<int64_t*>(pq), 8 * kOtaskStride, 0, s);
   if (e != hipSuccess) return e;
   if (nv <= 0) return hipSuccess;
   if (!p.orow || !p.oind || !p.prow) return hipErrorInvalidValue;
@@ -1336,7 +1471,7 @@ hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out
                                 : (one ? hash_fn<PV, OT, true>(o.groups) : hash_fn<PV, OT>(o.groups));
     hipLaunchKernelGGL(fn, dim3(nwg), dim3(256), 0, s, (const int64_t*)p.voff, (const int64_t*)p.orow,
                        (const int32_t*)p.oind, (const int32_t*)p.prow, nv, out,
-                       reinterpret_cast<unsigned long long*>(total), ctr, o.side, (int32_t)piece,
+                       reinterpret_cast<unsigned long long*>(total), ctr, pq, o.side, (int32_t)piece,
                        o.lookup == 2 ? 1 : 0, lmax, (const int32_t*)p.lrow, (const unsigned long long*)lq, lcap,
                        lq + 1);
     return hipGetLastError();  // the long rows were counted in its first phase
@@ -1344,8 +1479,8 @@ hipError_t launch_count_orient_t(const Prep& p, int64_t nv, int64_t nnz, OT* out
     const OrientFn<OT> fn = one ? orient_fn<PV, OT, true>(o.groups, o.window) : orient_fn<PV, OT>(o.groups, o.window);
     hipLaunchKernelGGL(fn, dim3(nwg), dim3(o.wg), orient_lds_bytes(o.wg / 64, o.window, PV), s,
                        (const int64_t*)p.voff, (const int64_t*)p.orow, (const int32_t*)p.oind,
-                       (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total), ctr, o.side,
-                       (int32_t)piece, lmax);
+                       (const int32_t*)p.prow, nv, out, reinterpret_cast<unsigned long long*>(total), ctr, pq,
+                       o.side, (int32_t)piece, lmax);
   }
   if (o.lng) {
     const LongFn<OT> lf = long_fn<PV, OT>(o.groups);
@@ -1414,7 +1549,7 @@ hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind
                            bool prepped) {
   if (!p.orow || !p.oind) return hipErrorInvalidValue;
   // task counter, huge rows listed and taken
-  hipError_t e = prepped ? hipSuccess : launch_fill_i64(p.dscal + 5, 3, 0, s);
+  hipError_t e = prepped ? hipSuccess : launch_fill_i64(p.dscal + 5, kScalars - 5, 0, s);
   if (e != hipSuccess || nv <= 0) return e;
   if (!p.huge || !p.odeg) return hipErrorInvalidValue;
   const int64_t cap = nnz / kHugeRow + 1;
@@ -1422,9 +1557,9 @@ hipError_t launch_orient_t(const Prep& p, int64_t nv, int64_t nnz, const IT* ind
     hipLaunchKernelGGL(k_huge_rows, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, (const int64_t*)p.voff, nv,
                        p.huge, cap, reinterpret_cast<unsigned long long*>(p.dscal + 6), p.odeg);
   // at most one wave per 64 vertices: ceil(nv / 256) workgroups of 4 waves, capped
-  hipLaunchKernelGGL(k_orient_rows<IT>, dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
-                     (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow,
-                     reinterpret_cast<unsigned long long*>(p.dscal + 5), (const int32_t*)p.huge, cap,
+  hipLaunchKernelGGL((k_orient_rows<IT, true>), dim3(grid_for(nv, 256, (int64_t)cus * 16)), dim3(256), 0, s,
+                     (const int64_t*)p.voff, ind, (const int32_t*)p.vbase, nv, p.oind, p.orow, p.otask,
+                     (const int32_t*)p.huge, cap,
                      reinterpret_cast<const unsigned long long*>(p.dscal + 6),
                      reinterpret_cast<unsigned long long*>(p.dscal + 7), (const int32_t*)p.odeg);
   return hipGetLastError();

@@ -15,8 +15,9 @@
 // streams N+ of the lower end of each arc, on average the shorter list
 // under degree order, but a hub row then has as many partners as the hub
 // has lower neighbours.  Work is therefore handed out in pieces of P
-// consecutive arc positions (from the counter ctr, or with ONE piece i for
-// wave i, when the grid has a wave for every piece); prow[b] is the row holding
+// consecutive arc positions (from eight counters pq, one per XCD, ticket n
+// of counter x being piece 8 n + x, or with ONE piece i for wave i, when
+// the grid has a wave for every piece); prow[b] is the row holding
 // position b * P, and a piece walks the rows it overlaps, taking the
 // partners that fall inside it, so a hub row spreads over many waves and a
 // run of short rows shares one.
@@ -131,7 +132,7 @@ template <int AW, int G, bool PF, int WPE, bool PV, typename OT, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_count_orient(
     const int64_t* __restrict__ voff, const int64_t* __restrict__ orow, const int32_t* __restrict__ oind,
     const int32_t* __restrict__ prow, int64_t nv, OT* __restrict__ out, unsigned long long* total,
-    unsigned long long* ctr, int side, int32_t P, int64_t lmax) {
+    unsigned long long* ctr, unsigned long long* pq, int side, int32_t P, int64_t lmax) {
   constexpr int CH = 8;
   extern __shared__ int32_t orient_lds[];
   const int lane = threadIdx.x & 63;
@@ -145,18 +146,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int64_t M = voff[nv];  // arc positions
   // side 0: partners are out-neighbours, 1: in-neighbours, 2: in-neighbours
   // unless the orientation listed rows of more than 4096 arcs (their count
-  // sits two words after the piece counter)
+  // is ctr[2])
   const bool by_in = side == 2 ? uniform64((int64_t)ctr[2]) == 0 : side != 0;
   // ONE: the grid has a wave for every piece (a small graph), and wave i
-  // takes piece i and nothing else: no counter, whose single address would
-  // otherwise serialise a few thousand waves' atomics.
+  // takes piece i and nothing else: no counter atomics at all.  Otherwise
+  // the pieces come from the per-XCD counters, which spread the atomics
+  // over eight addresses (one address serialises them, about 9 ns each).
+  const int64_t npieces = (M + P - 1) / P;
+  const int xcd = blockIdx.x & 7;
+  int qk = 0;
   for (;;) {
     int64_t b = 0;
     if (ONE) {
       b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
     } else {
-      if (lane == 0) b = (int64_t)atomicAdd(ctr, 1ull);
-      b = uniform64(__shfl(b, 0, 64));
+      b = next_task(pq, npieces, xcd, qk);
+      if (b < 0) break;
     }
     const int64_t a0 = b * P;
     if (a0 >= M) break;

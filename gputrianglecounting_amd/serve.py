@@ -169,7 +169,7 @@ def request_share(nv: int, nnz: int) -> int:
 
 
 BATCH_FIXED = 7 * 256  # a block's table row and region rounding (the admission policy's fixed part)
-SCRATCH_FIXED = 1040 * 256  # the scratch's fixed part (piece-list slack, counters, rounding)
+SCRATCH_FIXED = 1060 * 256  # the scratch's fixed part (piece-list slack, counters, rounding)
 BLOCK_FIXED = BATCH_FIXED + SCRATCH_FIXED  # what a serving block adds to its requests' shares
 
 

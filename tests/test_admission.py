@@ -180,7 +180,7 @@ def _orient_batch_scratch(nv, nnz):
     bounds, the orientation, piece rows, hub rows, degrees, long rows), each
     region rounded to 256 bytes."""
     r = lambda b: -(-b // 256) * 256  # noqa: E731
-    return (r(8 * (nv + 1)) + r(4 * (nv + 1)) + r(64) + r(96) + r(16 * (nv + 1)) + r(4 * (nnz + 1))
+    return (r(8 * (nv + 1)) + r(4 * (nv + 1)) + r(8 * 544) + r(96) + r(16 * (nv + 1)) + r(4 * (nnz + 1))
             + r(4 * (nnz // 256 + 2 + 65536)) + r(4 * (nnz // 4096 + 1)) + r(4 * (nv + 1)) + r(4 * (nnz // 512 + 2)))
 
 
