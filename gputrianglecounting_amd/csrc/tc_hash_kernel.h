@@ -57,18 +57,31 @@ __device__ __forceinline__ uint32_t cuckoo_mult(int i) {
 // instruction issue).  Ids of 2^24 and more share the buckets of their low
 // 24 bits: a window where that overfills a bucket pair fails its builds and
 // is laid out sorted instead, so counts never depend on it.  The scaling is
-// the high word of a 24 x 24-bit product (x times nb * 2^8).  HIP's __umul24
-// returns int: its product is taken as unsigned before any shift (a signed
-// shift of it gave out-of-range buckets).
+// the high word of a 24 x 24-bit product (x times nb * 2^8).
+//
+// Both products are written as the instructions themselves (mul_u24,
+// mulhi_u24): the compiler picks the 24-bit multiplies only when it can see,
+// in the same basic block, that both operands fit 24 bits, and the
+// multipliers come into the probe loops from outside them, so __umul24 and
+// the 64-bit product there became masks and quarter-rate 32-bit multiplies.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t c) {
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(c), "v"(a));  // low 32 bits of a[23:0] * c[23:0]
+  return r;
+}
+__device__ __forceinline__ uint32_t mulhi_u24(uint32_t a, uint32_t c) {
+  uint32_t r;
+  asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "s"(c), "v"(a));  // bits 32..47 of a[23:0] * c[23:0]
+  return r;
+}
 __device__ __forceinline__ uint32_t hash_bucket(int32_t key, uint32_t c, uint32_t nb) {
-  const uint32_t x = (uint32_t)__umul24((uint32_t)key, c) >> 8;
-  return (uint32_t)(((uint64_t)x * (uint64_t)((nb << 8) & 0xffffffu)) >> 32);
+  return mulhi_u24(mul_u24((uint32_t)key, c) >> 8, (nb << 8) & 0xffffffu);
 }
 
 // Bucket of an id among 2^(32 - shift) buckets: the top bits of its low
 // 24 bits times c (mod 2^32), one multiply and one shift (small tables).
 __device__ __forceinline__ uint32_t hash_bucket_p2(int32_t key, uint32_t c, uint32_t shift) {
-  return (uint32_t)__umul24((uint32_t)key, c) >> shift;
+  return mul_u24((uint32_t)key, c) >> shift;
 }
 
 // Slots of the table of a window of n ids on at most cap slots: nb buckets
