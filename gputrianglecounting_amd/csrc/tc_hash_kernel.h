@@ -161,16 +161,18 @@ __device__ __forceinline__ int32_t hash_probe(const int32_t (&e)[N], const HashT
 // the table by lane gl of a group of LG lanes: vectors gl, gl + LG, ...,
 // the next one in flight while the current one is probed.  Only a list's
 // first and last vectors hold elements of other lists, which are set to
-// kNoId.
+// kNoId.  The next vector is loaded whatever the trip count (the last one
+// again past the end), so no branch keeps the old value alive: a
+// conditional load cost four register copies per vector.
 template <int LG, bool XC, bool P2>
 __device__ __forceinline__ int32_t hash_stream_t(const int4* __restrict__ b4, int32_t nvec, int32_t lo, int32_t hi,
                                                  int gl, const HashTab& t, int32_t* X) {
   int32_t cnt = 0;
-  int4 nx = make_int4(kNoId, kNoId, kNoId, kNoId);
-  if (gl < nvec) nx = b4[gl];
+  if (gl >= nvec) return 0;
+  int4 nx = b4[gl];
   for (int32_t j = gl; j < nvec; j += LG) {
     const int4 a = nx;
-    if (j + LG < nvec) nx = b4[j + LG];
+    nx = b4[j + LG < nvec ? j + LG : nvec - 1];  // past the end: the last vector again (cached, unused)
     int32_t e[4] = {a.x, a.y, a.z, a.w};
     if ((uint32_t)(j - 1) >= (uint32_t)(nvec - 2)) {  // j == 0 or j == nvec - 1
 #pragma unroll
