@@ -337,6 +337,13 @@ def run_serve(tc, pool: int = 48, per_rate: int = 192, rel_rates=(0.25, 0.5, 0.7
         if r.total != alone[k][1] or not np.array_equal(r.out, alone[k][0]):
             bad += 1
 
+    import gc
+
+    # the collector's full passes over this process's heap (torch and numpy
+    # loaded) take tens of milliseconds: everything allocated so far is moved
+    # out of its reach, so a pause does not land inside a timed burst
+    gc.collect()
+    gc.freeze()
     try:
         # warm the slots' contexts, then one untimed burst: the server's
         # pinned pool is mapped into the process on first touch (~3 GB/s;

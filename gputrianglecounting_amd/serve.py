@@ -44,7 +44,7 @@ def _bind():
     L.tc_server_destroy.restype = C.c_int
     L.tc_server_destroy.argtypes = [vp]
     L.tc_server_submit.restype = C.c_int64
-    L.tc_server_submit.argtypes = [vp, C.c_int32, C.c_int64, _I32P, _I32P]
+    L.tc_server_submit.argtypes = [vp, C.c_int32, C.c_int64, vp, vp]
     L.tc_server_wait.restype = C.c_int
     L.tc_server_wait.argtypes = [vp, C.c_int64, _I32P, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_double)]
     L.tc_server_cancel.restype = C.c_int
@@ -91,8 +91,7 @@ class Server:
     def submit(self, g: Graph) -> int:
         off = np.ascontiguousarray(g.off, np.int32)
         ind = np.ascontiguousarray(g.ind, np.int32)
-        rid = _bind().tc_server_submit(self._h, g.n, g.nnz, off.ctypes.data_as(_I32P),
-                                       ind.ctypes.data_as(_I32P) if g.nnz else None)
+        rid = _bind().tc_server_submit(self._h, g.n, g.nnz, off.ctypes.data, ind.ctypes.data if g.nnz else None)
         if rid < 0:
             raise T.TCError("tc_server_submit rejected the request", int(rid))
         self._nv[rid] = g.n
