@@ -464,6 +464,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Rows of a task of up to kWalkRow arcs are oriented together, as one arc
 // range walked in chunks of 64 consecutive arcs (one per lane).
 constexpr int64_t kWalkRow = 1024;
+// Chunks of a task whose ids and degrees are loaded at once.
+constexpr int kWalkAhead = 4;
 // The next task (64 vertices from 64 t) for a wave of a workgroup on XCD
 // xcd, or -1 when every task is taken.  Task t belongs to counter t & 7
 // (otask[x * kOtaskStride]), whose n-th ticket is task 8 n + x; a wave takes
@@ -559,6 +561,14 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
       wcur[lane] = make_int2(0, 0);
       int64_t c0 = B;
       int32_t carry = 0;  // 1 + the lane whose row holds arc c0 when it started before c0
+      // ids (and, when the task's rows are all of one graph, their degrees)
+      // of kWalkAhead chunks loaded together from pc on, so that a chunk's
+      // two dependent loads are not waited for one chunk at a time
+      const int32_t vb0 = __builtin_amdgcn_readlane(vb, 0);
+      const bool one = vb0 == __builtin_amdgcn_readlane(vb, last);
+      bool have = false;
+      int64_t pc = 0;
+      int32_t pw[kWalkAhead], pd[kWalkAhead];
       while (c0 < E) {
         const uint64_t om = __ballot(obst && e > c0);  // the next row not walked
         int64_t os = INT64_MAX, oe = 0;
@@ -577,7 +587,31 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
         const int32_t n = (int32_t)(cend - c0);  // 1 .. 64 arcs
         const int32_t c0r = (int32_t)(c0 - B);
         const bool valid = lane < n;
-        const int32_t w = valid ? (int32_t)ind[c0 + lane] : 0;
+        const int64_t ahead = c0 - pc;
+        if (!have || ahead < 0 || ahead >= 64 * kWalkAhead || (ahead & 63)) {
+          have = true;
+          pc = c0;
+          const int64_t lim = os < E ? os : E;  // up to the next row stepped over
+#pragma unroll
+          for (int j = 0; j < kWalkAhead; ++j) {
+            const int64_t q = c0 + 64 * j + lane;
+            pw[j] = q < lim ? (int32_t)ind[q] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kWalkAhead; ++j) {
+            const int64_t q = c0 + 64 * j + lane;
+            pd[j] = one && q < lim ? deg[(int64_t)vb0 + pw[j]] : 0;
+          }
+        }
+        const int jj = (int)((c0 - pc) >> 6);  // the same in every lane
+        int32_t w = pw[0], dq = pd[0];
+#pragma unroll
+        for (int j = 1; j < kWalkAhead; ++j)
+          if (jj == j) {
+            w = pw[j];
+            dq = pd[j];
+          }
+        if (!valid) w = 0;
         wst[lane] = lane == 0 ? carry : 0;
         wave_lds_sync();
         const int32_t rel = (int32_t)(b - B) - c0r;  // this lane's row start within the chunk
@@ -587,7 +621,7 @@ __global__ __launch_bounds__(256) void k_orient_rows(const int64_t* __restrict__
         const int r = r1 > 0 ? r1 - 1 : 0;
         const int4 ri = wrow[r];
         const int2 cur = wcur[r];
-        const int32_t dw = valid ? deg[(int64_t)ri.w + w] : 0;
+        const int32_t dw = valid ? (one ? dq : deg[(int64_t)ri.w + w]) : 0;
         const bool keep = valid && orient_keep(ri.y, ri.z, dw, w);
         const bool other = valid && !keep;
         const uint64_t mk = __ballot(keep), mi = __ballot(other);
