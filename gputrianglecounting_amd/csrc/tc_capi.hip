@@ -1004,7 +1004,8 @@ struct Candidate {
 // The tuner's search, in stages, each around the fastest configuration so
 // far (best): 0 = partner side x partners per wave on the oriented scheme;
 // 1 = LDS window x workgroup size of its count kernel; 2 = piece size (the
-// partition granularity); 3 = lookup (walked array or table) x long rows;
+// partition granularity); 3 = lookup (walked array or table) x long rows,
+// and the table at 4, 8 and 16 partners per wave;
 // 4 = the edge-list scheme's streamed kernel variants x small-edge
 // threshold x level-1 ranges per workgroup.  The reference's counterpart
 // is its sweep over 8 block sizes x 6 threads per intersection
@@ -1050,6 +1051,15 @@ std::vector<tc_cfg> tune_stage(int stage, const tc_cfg& best) {
         c.long_rows = lr;
         out.push_back(c);
       }
+    // the table kernel at partners-per-wave settings of its own (stage 0
+    // searched them with the walked array)
+    for (int g : {4, 8, 16}) {
+      tc_cfg c = best;
+      c.lookup = 2;
+      c.long_rows = 2;
+      c.row_groups = g;
+      out.push_back(c);
+    }
     return out;
   }
   for (int i = 0; i < tcd::num_variants(); ++i) {
