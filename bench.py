@@ -111,16 +111,17 @@ def algorithmic_bytes(g, small_max: int, search_ratio: int) -> dict:
                 stream_units=stream_units)
 
 
-def algorithmic_bytes_oriented(g, by_in: bool = False) -> dict:
+def algorithmic_bytes_oriented(g, side: int = 0) -> dict:
     """Bytes of the oriented per-vertex count by definition.
 
     Arcs v -> w are kept iff (d(v), v) < (d(w), w).  Each oriented row is
     read once into the table (4 d+ bytes), and for every arc u -> w one
     partner list is streamed after its two row offsets (16 bytes): N+(w)
-    when rows take their out-neighbours as partners (the default for large
-    single graphs), N+(u) when they take their in-neighbours (``by_in``,
-    batches and smaller graphs).  Counts and the total are written once.
-    Reading the CSR to orient it is prep, outside the count kernel."""
+    when rows take their out-neighbours as partners (``side`` 0), N+(u) when
+    they take their in-neighbours (``side`` 1; ``side`` 2 takes them unless
+    the orientation has rows of more than 4096 arcs, as the count kernels
+    decide).  Counts and the total are written once.  Reading the CSR to
+    orient it is prep, outside the count kernel."""
     off = g.off.astype(np.int64)
     ind = g.ind
     d = np.diff(off)
@@ -138,6 +139,7 @@ def algorithmic_bytes_oriented(g, by_in: bool = False) -> dict:
         keep = (d[src] < d[dst]) | ((d[src] == d[dst]) & (src < dst))
         dplus += np.bincount(src[keep], minlength=g.n)
     arcs = int(dplus.sum())
+    by_in = side == 1 or (side == 2 and int(dplus.max(initial=0)) <= 4096)
     stream = 0
     for r0, r1 in blocks:
         src = np.repeat(np.arange(r0, r1, dtype=np.int64), d[r0:r1])
@@ -145,7 +147,8 @@ def algorithmic_bytes_oriented(g, by_in: bool = False) -> dict:
         keep = (d[src] < d[dst]) | ((d[src] == d[dst]) & (src < dst))
         stream += int(dplus[(src if by_in else dst)[keep]].sum())
     b = 4.0 * (arcs + stream) + 16.0 * arcs + 4 * g.n + 8
-    return dict(b_kernel=b, oriented_arcs=arcs, oriented_stream_elements=stream, max_out_degree=int(dplus.max()))
+    return dict(b_kernel=b, oriented_arcs=arcs, oriented_stream_elements=stream,
+                max_out_degree=int(dplus.max(initial=0)), partners="in-neighbours" if by_in else "out-neighbours")
 
 
 def cpu_baseline(g, target_s: float) -> dict:
@@ -297,7 +300,7 @@ def run_batch(tc, steps: int, warmup: int, G_: int = 4096) -> dict:
         ks.append(tc.last_timing().count_ms)
     tc.set_timing(False)
     k_ms = float(np.median(ks))
-    ab = algorithmic_bytes_oriented(batch_as_graph(b), by_in=True)
+    ab = algorithmic_bytes_oriented(batch_as_graph(b), side=1)
     roof = {"bound": "hbm", "achieved": round(ab["b_kernel"] / (k_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ab["b_kernel"] / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": ("k_count_hash" if cfg.get("lookup") in (2, 3) else "k_count_orient")
@@ -575,8 +578,12 @@ def main() -> int:
     # (the oriented count builds no edge list)
     edge_work = {} if oriented else dict(zip(("intersected_edges", "merge_work"), tc.last_edge_work()))
     ab = algorithmic_bytes(g, kcfg["small_max"], kcfg["search_ratio"] if kcfg["algo"] in (0, 3) else -1)
+    plan = None
     if oriented:
-        ab.update(algorithmic_bytes_oriented(g))
+        # the launch plan the count ran with (its partner side sets the bytes)
+        plan = tc.oriented_plan(g.n, g.nnz, cfg=cfg)
+        ab.update(algorithmic_bytes_oriented(g, side=plan["side"]))
+        plan["partners"] = ab["partners"]
     t_k = float(np.mean(kern)) / 1e3
     achieved = ab["b_kernel"] / t_k / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -617,7 +624,7 @@ def main() -> int:
         "data": "synthetic: seeded R-MAT/ER/RGG from the package's counter-based generators",
         "config": dict(config, n=g.n, arcs=g.nnz, undirected_edges=edges, **edge_work,
                        csr="int64" if args.int64 else "int32",
-                       kernel_cfg=tc.default_cfg() if cfg is None else cfg),
+                       kernel_cfg=tc.default_cfg() if cfg is None else cfg, kernel_plan=plan),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,

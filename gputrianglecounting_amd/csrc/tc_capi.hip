@@ -124,13 +124,18 @@ int resolve(const tc_ctx* ctx, const tc_cfg* in, tc_cfg* out) {
 // 4: 222), s20 8, the mixed 4096-graph batch 16, RGG n=4M 16-32.  A batch
 // is sized by its mean degree only: its lists are those of its (small)
 // graphs, whatever the batch's total.
-int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch);
+int orient_lookup(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch);
+int orient_side(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch);
+constexpr int64_t kOutArcs = int64_t(1) << 30;
 int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false) {
   if (cfg.row_groups > 0) return cfg.row_groups;
-  if (!batch && nnz >= (int64_t(1) << 30)) return 1;
-  // the table kernel: one partner per wave from 2^28 arcs (s24: 143.4 ms;
-  // 2: 144.8), two below (s22: 26.6 ms; 1: 27.7)
-  if (!batch && orient_lookup(cfg, nnz, batch) == 1) return nnz >= (int64_t(1) << 28) ? 1 : 2;
+  if (!batch && nnz >= kOutArcs) return 1;
+  // the table kernel: with in-neighbour partners four per wave (s24: 88.4
+  // ms; 2: 90.3, 8: 90.6, 1: 98.4; s21 and s20 the same order); with
+  // out-neighbour partners one from 2^28 arcs (s24: 143.4 ms; 2: 144.8),
+  // two below (s22: 26.6 ms; 1: 27.7)
+  if (!batch && orient_lookup(cfg, nv, nnz, batch) == 1)
+    return orient_side(cfg, nv, nnz, batch) == 1 ? 4 : nnz >= (int64_t(1) << 28) ? 1 : 2;
   if (!batch && nnz >= (int64_t(1) << 28)) return 2;
   const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
   if (dbar < 10) return 32;
@@ -150,17 +155,23 @@ Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Sc
 // Partner side of the oriented count (launch_count_orient's side): rows take
 // their in-neighbours as partners (streaming the lower end's row of each
 // arc) with oriented = 2 and out-neighbours with 1.  The automatic 3 takes
-// in-neighbours for batches and for single graphs below 2^24 arcs, and for
-// larger single graphs unless their orientation lists hub rows (more than
-// 4096 arcs, decided on the device).  Measured, in/out count time: the mixed
-// batch 33.1/36.6 ms, R-MAT s14 0.09/0.18, s18 1.84/2.33, RGG n=4M 1.73/1.97,
-// ER n=4M 1.86/1.98; but R-MAT s20 7.32/6.27 and s24 393/207 ms, where the
+// in-neighbours for batches, for single graphs below 2^24 arcs and for the
+// table kernel below 2^30 arcs; otherwise in-neighbours unless the
+// orientation lists hub rows (more than 4096 arcs, decided on the device).
+// Measured with the walked array, in/out count time: the mixed batch
+// 33.1/36.6 ms, R-MAT s14 0.09/0.18, s18 1.84/2.33, RGG n=4M 1.73/1.97, ER
+// n=4M 1.86/1.98; but R-MAT s20 7.32/6.27 and s24 393/207 ms, where the
 // hub-side lists streamed with out-neighbour partners stay in the caches.
+// With the table kernel (the build's displacement fix made its many small
+// builds cheap): s24 88.4/117.0 ms, s22 22.0/25.0, s21 12.9/13.9, s20
+// 6.05/6.76; but s26 (2.1e9 arcs) 1543/1097.
 constexpr int64_t kSideArcs = int64_t(1) << 24;
-int orient_side(const tc_cfg& cfg, bool batch, int64_t nnz) {
+int orient_side(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch) {
   if (cfg.oriented == 2) return 1;
   if (cfg.oriented != 3) return 0;
-  return batch || nnz < kSideArcs ? 1 : 2;
+  if (batch || nnz < kSideArcs) return 1;
+  if (orient_lookup(cfg, nv, nnz, batch) == 1 && nnz < kOutArcs) return 1;
+  return 2;
 }
 
 // How the oriented count meets a partner's list with the row's (the
@@ -169,15 +180,19 @@ int orient_side(const tc_cfg& cfg, bool batch, int64_t nnz) {
 // twice per element of wave-coalesced reads (k_count_hash), 2 = the same
 // kernel searching a sorted window per element.  cfg.lookup 1, 2 and 3 ask
 // for them; 0 (automatic) takes the table for single graphs from 3 * 2^25
-// arcs up and the walked array otherwise.  Measured count times, walked /
-// table (each with its best partners per wave): R-MAT s24 183.1/143.4 ms,
-// s22 (1.3e8 arcs) 34.5/26.5, s21 (6.4e7) 13.9/14.7, s20 6.3/6.6, RGG n=4M
-// 1.7/2.7, the mixed 4096-graph batch 32.5/40.5: the table's coalesced reads
-// pay off on long partner lists, its build on short rows does not.
+// arcs up, and from 2^24 arcs when the mean degree is at least 16, and the
+// walked array otherwise.  Measured count times, walked / table (the
+// table with in-neighbour partners, four per wave, below 2^30 arcs): R-MAT
+// s24 183.1/88.4 ms, s22 (1.3e8 arcs) 34.5/22.0, s21 (6.4e7) 13.9/12.9, s20
+// 6.13/6.05; but s18 1.83/2.24, RGG n=4M 1.64/1.92 (16 per wave) and ER
+// n=4M (mean degree 8) 1.58/1.81: on short rows the table's builds do not
+// pay for themselves.
 constexpr int64_t kTableArcs = int64_t(3) << 25;
-int orient_lookup(const tc_cfg& cfg, int64_t nnz, bool batch) {
+constexpr int64_t kTableDenseArcs = int64_t(1) << 24;
+int orient_lookup(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch) {
   if (cfg.lookup >= 1 && cfg.lookup <= 3) return cfg.lookup - 1;
-  return !batch && nnz >= kTableArcs ? 1 : 0;
+  if (batch) return 0;
+  return nnz >= kTableArcs || (nnz >= kTableDenseArcs && nnz >= 16 * nv) ? 1 : 0;
 }
 // Rows longer than a wave's LDS window counted by workgroups sharing one
 // table (the table kernel's long-row phase, or k_count_long after the walked
@@ -301,9 +316,9 @@ tcd::Prep make_prep(void* base, const Layout& L, bool batch) {
 tcd::OrientLaunch orient_launch(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch) {
   tcd::OrientLaunch o;
   o.groups = orient_groups(cfg, nv, nnz, batch);
-  o.side = orient_side(cfg, batch, nnz);
+  o.side = orient_side(cfg, nv, nnz, batch);
   o.piece = cfg.piece > 0 ? cfg.piece : batch ? tcd::kOrientPieceBatch : tcd::kOrientPiece;
-  o.lookup = orient_lookup(cfg, nnz, batch);
+  o.lookup = orient_lookup(cfg, nv, nnz, batch);
   o.lng = orient_long(cfg, nnz, batch);
   o.wg = cfg.orient_wg;
   o.window = cfg.window;
@@ -618,6 +633,23 @@ int tc_copy_i32(tc_ctx* c, int32_t* d_dst, const int32_t* d_src, int64_t n, void
   hipError_t e = hipSetDevice(c->device);
   if (e == hipSuccess) e = tcd::launch_copy_i32(d_dst, d_src, n, static_cast<hipStream_t>(stream));
   return e == hipSuccess ? TC_OK : hip_fail(c, e, "copy");
+}
+
+int tc_oriented_plan(tc_ctx* c, int64_t nv, int64_t nnz, int32_t batch, const tc_cfg* cfg_in, int32_t plan[4]) {
+  if (!c || !plan || nv < 0 || nnz < 0) return TC_EINVAL;
+  tc_cfg cfg;
+  if (resolve(c, cfg_in, &cfg) < 0) return set_err(c, TC_EINVAL, "unsupported launch configuration");
+  if (!cfg.oriented) {
+    plan[0] = plan[1] = -1;
+    plan[2] = plan[3] = 0;
+    return TC_OK;
+  }
+  const tcd::OrientLaunch o = orient_launch(cfg, nv, nnz, batch != 0);
+  plan[0] = o.lookup;
+  plan[1] = o.side;
+  plan[2] = o.groups;
+  plan[3] = o.lng ? 1 : 0;
+  return TC_OK;
 }
 
 int tc_last_edge_work(tc_ctx* c, int64_t out[2]) {

@@ -116,6 +116,7 @@ def lib() -> C.CDLL:
         "tc_count_vertices_lim_i32": (C.c_int, [vp, i32, i64, vp, vp, i32, vp, i32, vp, vp, vp, vp]),
         "tc_device_cus": (C.c_int, [vp]),
         "tc_last_edge_work": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "tc_oriented_plan": (C.c_int, [vp, i64, i64, C.c_int32, C.POINTER(TcCfg), C.POINTER(C.c_int32)]),
         "tc_copy_i32": (C.c_int, [vp, vp, vp, i64, vp]),
         "tc_batch_count_pergraph_i32": (C.c_int, [vp, i32, C.POINTER(C.c_int64), vp, vp, vp, vp, C.POINTER(TcCfg), vp]),
         "tc_count_capture_i32": (C.c_int, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(TcCfg), C.POINTER(vp)]),
@@ -142,7 +143,7 @@ EXPORTS = ["tc_ctx_create", "tc_ctx_destroy", "tc_last_error", "tc_scratch_bytes
            "tc_validate_i32",
            "tc_batch_count_i32",
            "tc_batch_scratch_bytes", "tc_count_total_i32", "tc_count_vertices_i32",
-           "tc_count_vertices_lim_i32", "tc_device_cus", "tc_last_edge_work", "tc_copy_i32",
+           "tc_count_vertices_lim_i32", "tc_device_cus", "tc_last_edge_work", "tc_oriented_plan", "tc_copy_i32",
            "tc_batch_count_pergraph_i32", "tc_count_capture_i32", "tc_batch_capture_i32", "tc_graph_launch",
            "tc_graph_destroy", "tc_autotune_i32", "tc_autotune_batch_i32", "tc_server_create", "tc_server_destroy",
            "tc_server_submit", "tc_server_wait", "tc_server_cancel", "tc_server_get_stats"]
@@ -239,6 +240,13 @@ class TriangleCounter:
         buf = (C.c_int64 * 2)()
         self._check(lib().tc_last_edge_work(self._h, buf), "edge work")
         return int(buf[0]), int(buf[1])
+
+    def oriented_plan(self, nv: int, nnz: int, batch: bool = False, cfg=None) -> dict:
+        """The oriented count's resolved launch plan (tc_oriented_plan):
+        lookup, partner side, partners per wave, long rows by workgroups."""
+        buf = (C.c_int32 * 4)()
+        self._check(lib().tc_oriented_plan(self._h, nv, nnz, 1 if batch else 0, _cfgp(cfg), buf), "oriented plan")
+        return dict(zip(("lookup", "side", "groups", "long_rows"), (int(x) for x in buf)))
 
     def reserve(self, nbytes: int) -> None:
         self._check(lib().tc_ctx_reserve(self._h, nbytes), "reserve")

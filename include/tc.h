@@ -70,10 +70,12 @@ typedef struct tc_ctx tc_ctx;
  *                  of the input rows, each triangle found once and its three
  *                  vertices credited; 2 = the same, but a row takes its
  *                  in-neighbours as partners (streams the lower end's row of
- *                  each arc); 3 (default) = automatic: 2 for batches and
- *                  for graphs below 2^24 arcs, and for larger graphs unless
- *                  they have rows of more than 4096 arcs (then 1, decided
- *                  on the device); -1 = every undirected edge's full lists
+ *                  each arc); 3 (default) = automatic: 2 for batches, for
+ *                  graphs below 2^24 arcs and for the table lookup below
+ *                  2^30 arcs; otherwise 2 unless the orientation has rows
+ *                  of more than 4096 arcs (then 1, decided on the device);
+ *                  tc_oriented_plan reports the choice; -1 = every
+ *                  undirected edge's full lists
  *                  (the reference's scheme, the edge-list kernels).  Same
  *                  counts either way; other algos always use the full lists.
  *                  tc_last_edge_work reports (0, 0) for the oriented scheme,
@@ -81,7 +83,8 @@ typedef struct tc_ctx tc_ctx;
  *   row_groups     oriented scheme: partners of a row a wave streams at once,
  *                  each by 64/row_groups lanes (1, 2, 4, 8, 16 or 32;
  *                  default -1 = chosen from the size and mean degree: 1
- *                  from 2^30 arcs up; single graphs with the table lookup 1
+ *                  for single graphs from 2^30 arcs up; single graphs with
+ *                  the table lookup 4 with in-neighbour partners, else 1
  *                  from 2^28 arcs and 2 below; with the walked array 2 from
  *                  2^28 arcs; else 32 below mean degree 10, 16 below 20, 8
  *                  above)
@@ -92,7 +95,8 @@ typedef struct tc_ctx tc_ctx;
  *                  probes twice (k_count_hash); 3 = the same reads, each
  *                  element binary-searched in the sorted window;
  *                  0 (default) = automatic: 2 for single graphs from
- *                  3 * 2^25 arcs up, else 1
+ *                  3 * 2^25 arcs up, and from 2^24 arcs with a mean degree
+ *                  of at least 16; else 1
  *   long_rows      oriented scheme: 2 = rows whose N+ is longer than one
  *                  wave's LDS window (`window` ids for lookup 1, 624 for 2
  *                  and 3) are counted by whole workgroups sharing one table of
@@ -336,6 +340,19 @@ int tc_count_total_i32(tc_ctx* ctx, int32_t nv, int64_t nnz, const int32_t* d_of
 /* The same for an int64 CSR (as tc_count_i64: any nnz, ids below 2^31). */
 int tc_count_total_i64(tc_ctx* ctx, int64_t nv, int64_t nnz, const int64_t* d_off, const int64_t* d_ind,
                        int64_t* d_total, void* stream);
+
+/* The oriented count's launch plan for a count of nv vertices and nnz arcs
+ * (a batch's totals when batch != 0) under cfg (NULL: the context's
+ * default), with every automatic choice resolved: plan[0] = lookup (0 the
+ * walked array, k_count_orient; 1 the LDS table, k_count_hash; 2 the table
+ * kernel with sorted windows), plan[1] = partner side (0 out-neighbours, 1
+ * in-neighbours, 2 in-neighbours unless the orientation lists rows of more
+ * than 4096 arcs, decided on the device), plan[2] = partners per wave,
+ * plan[3] = 1 when rows longer than a wave's table are counted by
+ * workgroups.  The edge-list scheme gives {-1, -1, 0, 0}.  Launches
+ * nothing.  No reference counterpart (the reference has one kernel shape
+ * per launch). */
+int tc_oriented_plan(tc_ctx* ctx, int64_t nv, int64_t nnz, int32_t batch, const tc_cfg* cfg, int32_t plan[4]);
 
 /* Edge count E and total merge work W = sum over intersected edges of
  * |N(u)| + |N(w)| of the last edge-list-scheme count on this context

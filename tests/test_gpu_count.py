@@ -250,6 +250,31 @@ def test_oriented_only_for_algo_0(counter):
         _check(counter, g, cfg=dict(oriented=1, algo=algo))
 
 
+def test_oriented_plan(counter):
+    """tc_oriented_plan resolves the automatic choices by size and mean
+    degree, keeps explicit ones, and reports the edge scheme as such."""
+    plan = counter.oriented_plan
+    # R-MAT s24 ef16 (2^24 vertices, 2^29 arcs): the table, in-neighbours, 4 per wave, long rows by workgroups
+    assert plan(1 << 24, 1 << 29) == dict(lookup=1, side=1, groups=4, long_rows=1)
+    # 2^31 arcs: out-neighbour partners decided on the device, one per wave
+    assert plan(1 << 26, 1 << 31) == dict(lookup=1, side=2, groups=1, long_rows=1)
+    # dense from 2^24 arcs takes the table; sparse of the same size the walked array
+    assert plan(1 << 20, 1 << 25)["lookup"] == 1
+    p = plan(1 << 22, 1 << 25)
+    assert (p["lookup"], p["side"]) == (0, 2)
+    # small graphs and batches: the walked array with in-neighbour partners, groups by mean degree
+    assert plan(1 << 14, 1 << 18) == dict(lookup=0, side=1, groups=16, long_rows=0)
+    assert plan(1 << 21, 1 << 25, batch=True) == dict(lookup=0, side=1, groups=16, long_rows=0)
+    assert plan(1 << 18, 1 << 25, batch=True)["groups"] == 8
+    assert plan(1 << 20, 1 << 22)["groups"] == 32
+    # explicit settings are kept
+    assert plan(1 << 24, 1 << 29, cfg=dict(oriented=1, row_groups=8, lookup=1, long_rows=1)) == \
+        dict(lookup=0, side=0, groups=8, long_rows=0)
+    assert plan(1 << 24, 1 << 29, cfg=dict(oriented=-1)) == dict(lookup=-1, side=-1, groups=0, long_rows=0)
+    with pytest.raises(T.TCError):
+        plan(-1, 0)
+
+
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: "x".join(map(str, v)))
 @pytest.mark.parametrize("small_max", [16, 300, 2048, 0])
 @pytest.mark.parametrize("ranges", [1, 32])
