@@ -119,8 +119,8 @@ def algorithmic_bytes_oriented(g, side: int = 0) -> dict:
     partner list is streamed after its two row offsets (16 bytes): N+(w)
     when rows take their out-neighbours as partners (``side`` 0), N+(u) when
     they take their in-neighbours (``side`` 1; ``side`` 2 takes them unless
-    the orientation has rows of more than 4096 arcs, as the count kernels
-    decide).  Counts and the total are written once.  Reading the CSR to
+    the input has rows of more than 4096 arcs, as the count kernels decide
+    from the prep's hub-row list).  Counts and the total are written once.  Reading the CSR to
     orient it is prep, outside the count kernel."""
     off = g.off.astype(np.int64)
     ind = g.ind
@@ -139,7 +139,7 @@ def algorithmic_bytes_oriented(g, side: int = 0) -> dict:
         keep = (d[src] < d[dst]) | ((d[src] == d[dst]) & (src < dst))
         dplus += np.bincount(src[keep], minlength=g.n)
     arcs = int(dplus.sum())
-    by_in = side == 1 or (side == 2 and int(dplus.max(initial=0)) <= 4096)
+    by_in = side == 1 or (side == 2 and int(d.max(initial=0)) <= 4096)
     stream = 0
     for r0, r1 in blocks:
         src = np.repeat(np.arange(r0, r1, dtype=np.int64), d[r0:r1])

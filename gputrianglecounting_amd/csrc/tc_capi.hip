@@ -157,7 +157,8 @@ Scheme scheme_of(const tc_cfg& cfg) { return cfg.oriented ? Scheme::kOrient : Sc
 // arc) with oriented = 2 and out-neighbours with 1.  The automatic 3 takes
 // in-neighbours for batches, for single graphs below 2^24 arcs and for the
 // table kernel below 2^30 arcs; otherwise in-neighbours unless the
-// orientation lists hub rows (more than 4096 arcs, decided on the device).
+// prep lists hub rows (input rows of more than 4096 arcs, decided on the
+// device).
 // Measured with the walked array, in/out count time: the mixed batch
 // 33.1/36.6 ms, R-MAT s14 0.09/0.18, s18 1.84/2.33, RGG n=4M 1.73/1.97, ER
 // n=4M 1.86/1.98; but R-MAT s20 7.32/6.27 and s24 393/207 ms, where the

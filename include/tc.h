@@ -72,8 +72,8 @@ typedef struct tc_ctx tc_ctx;
  *                  in-neighbours as partners (streams the lower end's row of
  *                  each arc); 3 (default) = automatic: 2 for batches, for
  *                  graphs below 2^24 arcs and for the table lookup below
- *                  2^30 arcs; otherwise 2 unless the orientation has rows
- *                  of more than 4096 arcs (then 1, decided on the device);
+ *                  2^30 arcs; otherwise 2 unless the graph has rows of
+ *                  more than 4096 arcs (then 1, decided on the device);
  *                  tc_oriented_plan reports the choice; -1 = every
  *                  undirected edge's full lists
  *                  (the reference's scheme, the edge-list kernels).  Same
@@ -346,8 +346,8 @@ int tc_count_total_i64(tc_ctx* ctx, int64_t nv, int64_t nnz, const int64_t* d_of
  * default), with every automatic choice resolved: plan[0] = lookup (0 the
  * walked array, k_count_orient; 1 the LDS table, k_count_hash; 2 the table
  * kernel with sorted windows), plan[1] = partner side (0 out-neighbours, 1
- * in-neighbours, 2 in-neighbours unless the orientation lists rows of more
- * than 4096 arcs, decided on the device), plan[2] = partners per wave,
+ * in-neighbours, 2 in-neighbours unless the input has rows of more than
+ * 4096 arcs, decided on the device), plan[2] = partners per wave,
  * plan[3] = 1 when rows longer than a wave's table are counted by
  * workgroups.  The edge-list scheme gives {-1, -1, 0, 0}.  Launches
  * nothing.  No reference counterpart (the reference has one kernel shape
