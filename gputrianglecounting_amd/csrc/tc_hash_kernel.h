@@ -331,6 +331,21 @@ __device__ __forceinline__ HashTab hash_build_wg(const int32_t* __restrict__ src
 // walked in windows of 3 TSW / 4 ids with every N+(p) cut to the window's
 // range, as in the per-wave kernels.
 constexpr int64_t kLongRun = 16384;
+// Keys of the wave's 64 lanes sorted in descending order (bitonic network:
+// lane l ends with the l-th largest key).
+__device__ __forceinline__ uint32_t wave_sort_desc(uint32_t key, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)key, j, 64);
+      const bool keep_max = ((lane & j) == 0) == ((lane & k) == 0);
+      key = keep_max ? (key > o ? key : o) : (key < o ? key : o);
+    }
+  }
+  return key;
+}
+
 // The long-row phase: the workgroup takes listed long rows from lctr until
 // none is left, counting each with one table of TSW slots in K (and 16-bit
 // counters in X) shared by its four waves; returns this wave's matches.
@@ -512,17 +527,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           constexpr int LG = 64 / G;
           const int gid = lane / LG, gl = lane % LG;
           int64_t cg = 0;
-          for (int64_t j = 0; j < lt; j += G) {
-            const int64_t jj = j + gid;
-            int32_t c = 0;
-            if (jj < lt) {
-              const int32_t w = oind[tu + jj];
-              const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
-              c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
-              c = group_sum32<LG>(c);  // G = 1: every lane is here (j is the same in all)
-              if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+          if (G > 1) {
+            // The row's partners 64 at a time, one per lane (its id and N+
+            // bounds), ordered by list length, longest first: the G lists
+            // streamed together then take about as many steps, where in
+            // the CSR's order the longest of each G set the pace for all.
+            for (int64_t j0 = 0; j0 < lt; j0 += 64) {
+              const int32_t nb = (int32_t)(lt - j0 < 64 ? lt - j0 : 64);
+              int32_t pw = 0;
+              int64_t po = 0, pe = 0;
+              if (lane < nb) {
+                pw = oind[tu + j0 + lane];
+                po = orow[2 * pw];
+                pe = orow[2 * pw + 1];
+              }
+              const int64_t len = pe > po ? pe - po : 0;
+              const int64_t kl = len < (int64_t(1) << 25) - 2 ? len + 1 : (int64_t(1) << 25) - 1;
+              const uint32_t key = lane < nb ? ((uint32_t)kl << 6) | (uint32_t)lane : 0u;  // distinct, above the 0 of lanes past nb
+              const int src = (int)(wave_sort_desc(key, lane) & 63u);
+              pw = __shfl(pw, src, 64);
+              po = __shfl(po, src, 64);
+              pe = __shfl(pe, src, 64);
+              for (int32_t b = 0; b < nb; b += G) {
+                const int32_t si = b + gid;
+                const int32_t w = __shfl(pw, si & 63, 64);
+                const int64_t ow = __shfl(po, si & 63, 64), we = __shfl(pe, si & 63, 64);
+                int32_t c = 0;
+                if (si < nb) {
+                  c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
+                  c = group_sum32<LG>(c);
+                  if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+                }
+                cg += c;
+              }
             }
-            cg += c;
+          } else {
+            for (int64_t j = 0; j < lt; ++j) {
+              const int32_t w = oind[tu + j];
+              const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
+              int32_t c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
+              c = group_sum32<LG>(c);  // every lane is here (j is the same in all)
+              if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+              cg += c;
+            }
           }
           if (G == 1) {
             cu += uniform64(cg);  // every lane holds the same sums
