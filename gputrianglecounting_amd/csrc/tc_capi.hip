@@ -130,12 +130,13 @@ constexpr int64_t kOutArcs = int64_t(1) << 30;
 int orient_groups(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool batch = false) {
   if (cfg.row_groups > 0) return cfg.row_groups;
   if (!batch && nnz >= kOutArcs) return 1;
-  // the table kernel: with in-neighbour partners four per wave (s24: 88.4
-  // ms; 2: 90.3, 8: 90.6, 1: 98.4; s21 and s20 the same order); with
-  // out-neighbour partners one from 2^28 arcs (s24: 143.4 ms; 2: 144.8),
-  // two below (s22: 26.6 ms; 1: 27.7)
+  // the table kernel: with in-neighbour partners eight per wave, the
+  // partners of a row ordered by list length (s24: 84.0 ms; 4: 87.1; s22
+  // 21.4 / 21.6, s20 5.47 / 5.46; before the ordering 4 was best, s24 88.4
+  // ms; 2: 90.3, 8: 90.6, 1: 98.4); with out-neighbour partners one from
+  // 2^28 arcs (s24: 143.4 ms; 2: 144.8), two below (s22: 26.6 ms; 1: 27.7)
   if (!batch && orient_lookup(cfg, nv, nnz, batch) == 1)
-    return orient_side(cfg, nv, nnz, batch) == 1 ? 4 : nnz >= (int64_t(1) << 28) ? 1 : 2;
+    return orient_side(cfg, nv, nnz, batch) == 1 ? 8 : nnz >= (int64_t(1) << 28) ? 1 : 2;
   if (!batch && nnz >= (int64_t(1) << 28)) return 2;
   const double dbar = nv > 0 ? (double)nnz / (double)nv : 0.0;
   if (dbar < 10) return 32;

@@ -527,13 +527,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
           constexpr int LG = 64 / G;
           const int gid = lane / LG, gl = lane % LG;
           int64_t cg = 0;
-          if (G > 1) {
+          if (G > 1 && G <= 8) {
             // The row's partners 64 at a time, one per lane (its id and N+
             // bounds), ordered by list length, longest first: the G lists
             // streamed together then take about as many steps, where in
             // the CSR's order the longest of each G set the pace for all.
+            // (With 16 or 32 partners per wave the lists are short, and
+            // ordering them cost more than it saved: the 4096-graph batch
+            // at 16 per wave went from 32.0 to 41.5 ms.)
             for (int64_t j0 = 0; j0 < lt; j0 += 64) {
               const int32_t nb = (int32_t)(lt - j0 < 64 ? lt - j0 : 64);
+              if (nb <= G) {  // one set of G: the order changes nothing
+                int32_t c = 0;
+                if (gid < nb) {
+                  const int32_t w = oind[tu + j0 + gid];
+                  const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
+                  c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
+                  c = group_sum32<LG>(c);
+                  if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+                }
+                cg += c;
+                continue;
+              }
               int32_t pw = 0;
               int64_t po = 0, pe = 0;
               if (lane < nb) {
@@ -562,12 +577,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
               }
             }
           } else {
-            for (int64_t j = 0; j < lt; ++j) {
-              const int32_t w = oind[tu + j];
-              const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
-              int32_t c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
-              c = group_sum32<LG>(c);  // every lane is here (j is the same in all)
-              if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+            for (int64_t j = 0; j < lt; j += G) {
+              const int64_t jj = j + gid;
+              int32_t c = 0;
+              if (jj < lt) {
+                const int32_t w = oind[tu + jj];
+                const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
+                c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
+                c = group_sum32<LG>(c);  // G = 1: every lane is here (j is the same in all)
+                if (PV && gl == 0 && c) out_add(out, w, 2 * c);
+              }
               cg += c;
             }
           }

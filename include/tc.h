@@ -84,7 +84,7 @@ typedef struct tc_ctx tc_ctx;
  *                  each by 64/row_groups lanes (1, 2, 4, 8, 16 or 32;
  *                  default -1 = chosen from the size and mean degree: 1
  *                  for single graphs from 2^30 arcs up; single graphs with
- *                  the table lookup 4 with in-neighbour partners, else 1
+ *                  the table lookup 8 with in-neighbour partners, else 1
  *                  from 2^28 arcs and 2 below; with the walked array 2 from
  *                  2^28 arcs; else 32 below mean degree 10, 16 below 20, 8
  *                  above)

@@ -254,8 +254,8 @@ def test_oriented_plan(counter):
     """tc_oriented_plan resolves the automatic choices by size and mean
     degree, keeps explicit ones, and reports the edge scheme as such."""
     plan = counter.oriented_plan
-    # R-MAT s24 ef16 (2^24 vertices, 2^29 arcs): the table, in-neighbours, 4 per wave, long rows by workgroups
-    assert plan(1 << 24, 1 << 29) == dict(lookup=1, side=1, groups=4, long_rows=1)
+    # R-MAT s24 ef16 (2^24 vertices, 2^29 arcs): the table, in-neighbours, 8 per wave, long rows by workgroups
+    assert plan(1 << 24, 1 << 29) == dict(lookup=1, side=1, groups=8, long_rows=1)
     # 2^31 arcs: out-neighbour partners decided on the device, one per wave
     assert plan(1 << 26, 1 << 31) == dict(lookup=1, side=2, groups=1, long_rows=1)
     # dense from 2^24 arcs takes the table; sparse of the same size the walked array
