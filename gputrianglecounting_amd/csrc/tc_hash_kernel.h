@@ -549,26 +549,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 cg += c;
                 continue;
               }
-              int32_t pw = 0;
-              int64_t po = 0, pe = 0;
+              uint32_t key = 0u;  // lanes past nb sort last
               if (lane < nb) {
-                pw = oind[tu + j0 + lane];
-                po = orow[2 * pw];
-                pe = orow[2 * pw + 1];
+                const int32_t pw = oind[tu + j0 + lane];
+                const int64_t len = orow[2 * pw + 1] - orow[2 * pw];
+                const int64_t kl = len < 0 ? 1 : len < (int64_t(1) << 25) - 2 ? len + 1 : (int64_t(1) << 25) - 1;
+                key = ((uint32_t)kl << 6) | (uint32_t)lane;  // distinct, above 0
               }
-              const int64_t len = pe > po ? pe - po : 0;
-              const int64_t kl = len < (int64_t(1) << 25) - 2 ? len + 1 : (int64_t(1) << 25) - 1;
-              const uint32_t key = lane < nb ? ((uint32_t)kl << 6) | (uint32_t)lane : 0u;  // distinct, above the 0 of lanes past nb
-              const int src = (int)(wave_sort_desc(key, lane) & 63u);
-              pw = __shfl(pw, src, 64);
-              po = __shfl(po, src, 64);
-              pe = __shfl(pe, src, 64);
+              // only the order is kept; each set reloads its partners' ids
+              // and bounds (cached), which holds fewer registers than
+              // carrying them through the loop
+              const int order = (int)(wave_sort_desc(key, lane) & 63u);
               for (int32_t b = 0; b < nb; b += G) {
                 const int32_t si = b + gid;
-                const int32_t w = __shfl(pw, si & 63, 64);
-                const int64_t ow = __shfl(po, si & 63, 64), we = __shfl(pe, si & 63, 64);
+                const int src = __shfl(order, si & 63, 64);
                 int32_t c = 0;
                 if (si < nb) {
+                  const int32_t w = oind[tu + j0 + src];
+                  const int64_t ow = orow[2 * w], we = orow[2 * w + 1];
                   c = hash_stream<LG, PV>(ind4, ow + sh, we + sh, gl, tab, X);
                   c = group_sum32<LG>(c);
                   if (PV && gl == 0 && c) out_add(out, w, 2 * c);
