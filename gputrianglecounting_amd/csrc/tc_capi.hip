@@ -319,8 +319,12 @@ tcd::OrientLaunch orient_launch(const tc_cfg& cfg, int64_t nv, int64_t nnz, bool
   tcd::OrientLaunch o;
   o.groups = orient_groups(cfg, nv, nnz, batch);
   o.side = orient_side(cfg, nv, nnz, batch);
-  o.piece = cfg.piece > 0 ? cfg.piece : batch ? tcd::kOrientPieceBatch : tcd::kOrientPiece;
   o.lookup = orient_lookup(cfg, nv, nnz, batch);
+  // pieces of 512 arcs for the table kernel with in-neighbour partners from
+  // 2^26 arcs (count ms, pieces of 256/512: R-MAT s22 20.9/18.5, s24
+  // 82.1/82.1; s20 5.32/5.38 keeps 256)
+  const bool wide = !batch && o.lookup >= 1 && o.side == 1 && nnz >= (int64_t(1) << 26);
+  o.piece = cfg.piece > 0 ? cfg.piece : batch ? tcd::kOrientPieceBatch : wide ? 2 * tcd::kOrientPiece : tcd::kOrientPiece;
   o.lng = orient_long(cfg, nnz, batch);
   o.wg = cfg.orient_wg;
   o.window = cfg.window;

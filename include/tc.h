@@ -112,7 +112,9 @@ typedef struct tc_ctx tc_ctx;
  *                  count kernel (128 or 256, default 256)
  *   piece          oriented scheme: arc positions per work piece, the unit
  *                  the count kernel's waves take from a counter (256..16383;
- *                  default -1 = 256 for single graphs, 1024 for batches; a
+ *                  default -1 = 256 for single graphs (512 on the table
+ *                  lookup with in-neighbour partners from 2^26 arcs),
+ *                  1024 for batches; a
  *                  graph with fewer pieces than waves takes smaller ones) */
 typedef struct tc_cfg {
   int32_t wg_size;
